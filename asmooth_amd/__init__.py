@@ -1,0 +1,325 @@
+"""asmooth_amd: MI355X (gfx950) adaptive-smoothing path of asmooth.
+
+Python mirror of the reference's hot-path interface over the C ABI of
+``include/asmooth.h`` (``libasmooth.so``, loaded with ctypes).  The method
+names follow the reference functions they replace:
+
+    ctx = Context()                           # one HIP stream, device buffers
+    ctx.set_particles(p, np_local, boxsize)   # Particles after the exchange
+    ctx.construct(-bf*L, (1+bf)*L)            # KDTree::construct
+    ctx.neighbor_search(32)                   # neighbor_search loop
+    ctx.mark_halo_particles(halos, lo, hi)    # mark_halo_particles loop
+    ctx.assign_on_mesh(nc, sign=-1)           # clear_mesh + assign_on_mesh
+    ctx.mesh_total(nc)                        # mesh_total for n and halo
+    ctx.normalize_mesh(nc); ctx.get_mesh(nc)  # write_mesh_separate's normalisation
+
+Errors raise AsmoothError with the library's message; nothing falls back to
+a CPU path.  A missing library raises at import of this package's Context.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Sequence
+
+import numpy as np
+
+__all__ = [
+    "PARTICLE_DTYPE", "HALO_DTYPE", "K_SLOTS", "AsmoothError", "Context", "lib",
+    "build_buffered_set", "build_node_share", "read_xv", "write_xv", "write_mesh_separate",
+    "particles_from_xv", "timer_names",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libasmooth.so")
+
+# basic_types.h:10-15 (36 bytes) and :17-20 (20 bytes).
+PARTICLE_DTYPE = np.dtype([("x", "<f4", (3,)), ("v", "<f4", (3,)), ("dens", "<f4"),
+                           ("rk", "<f4"), ("igrp", "<i4")])
+HALO_DTYPE = np.dtype([("x", "<f4", (3,)), ("r", "<f4"), ("mass", "<f4")])
+assert PARTICLE_DTYPE.itemsize == 36 and HALO_DTYPE.itemsize == 20
+
+K_SLOTS = 33  # neighbour slots including the particle itself (k_neighbors.h:47)
+
+OPT_KEEP_NEIGHBORS = 1
+OPT_PROFILE = 2
+OPT_LEAF_SIZE = 3
+OPT_TILE = 4
+
+
+class AsmoothError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libasmooth.so (built by asmooth_amd.build); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AsmoothError(f"{LIB_PATH} is missing: run `python -m asmooth_amd.build` "
+                           "(the HIP library is required; there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f32, f64 = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_double
+    P = C.POINTER
+    sigs = {
+        "asm_version": (C.c_char_p, []),
+        "asm_create": (i32, [P(vp), i32]),
+        "asm_destroy": (None, [vp]),
+        "asm_last_error": (C.c_char_p, [vp]),
+        "asm_set_option": (i32, [vp, i32, i32]),
+        "asm_synchronize": (i32, [vp]),
+        "asm_set_particles": (i32, [vp, vp, i64, i64, f32]),
+        "asm_set_particles_device": (i32, [vp, vp, i64, i64, f32]),
+        "asm_tree_construct": (i32, [vp, f32, f32, P(i64)]),
+        "asm_neighbor_search": (i32, [vp, i32]),
+        "asm_mark_halo_particles": (i32, [vp, vp, i64, f32, f32, P(i64), P(f64)]),
+        "asm_assign_on_mesh": (i32, [vp, i32, i32]),
+        "asm_assign_on_meshes": (i32, [vp, P(i32), i32, i32]),
+        "asm_assign_on_global_mesh": (i32, [vp, i32, i32, P(i32)]),
+        "asm_mesh_total": (i32, [vp, i32, P(f64)]),
+        "asm_mesh_stats": (i32, [vp, i32, P(f64)]),
+        "asm_normalize_mesh": (i32, [vp, i32]),
+        "asm_get_mesh": (i32, [vp, i32, vp]),
+        "asm_mesh_device_ptr": (vp, [vp, i32]),
+        "asm_get_particles": (i32, [vp, vp]),
+        "asm_get_rk_dens": (i32, [vp, vp, vp, vp]),
+        "asm_get_neighbors": (i32, [vp, vp]),
+        "asm_build_buffered_set": (i64, [vp, i64, f32, f32, vp, i64, P(i64)]),
+        "asm_build_node_share": (i64, [vp, i64, f32, i32, P(i32), f32, vp, i64, P(i64)]),
+        "asm_read_xv": (i32, [C.c_char_p, P(vp), P(i64), vp]),
+        "asm_write_xv": (i32, [C.c_char_p, vp, i64, vp]),
+        "asm_free": (None, [vp]),
+        "asm_write_mesh_separate": (i32, [C.c_char_p, i32, vp, i32]),
+        "asm_timer_count": (i32, []),
+        "asm_timer_name": (C.c_char_p, [i32]),
+        "asm_get_timer": (i32, [vp, i32, P(f64), P(i64)]),
+        "asm_reset_timers": (i32, [vp]),
+        "asm_get_search_stats": (i32, [vp, P(C.c_uint64)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def timer_names() -> list[str]:
+    L = lib()
+    return [L.asm_timer_name(i).decode() for i in range(L.asm_timer_count())]
+
+
+class Context:
+    """One device context: the replacement of the reference's per-process
+    Particles / KDTree / mesh buffers (clumping_tree_main.cpp:178-216)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.asm_create(C.byref(h), int(device))
+        if rc != 0:
+            raise AsmoothError(f"asm_create(device={device}) failed with status {rc}")
+        self._h = h
+        self.n = 0
+        self.np_local = 0
+        self.boxsize = 0.0
+
+    # -- plumbing -------------------------------------------------------
+    def _check(self, rc: int, what: str) -> None:
+        if rc < 0:
+            msg = self._L.asm_last_error(self._h).decode(errors="replace")
+            raise AsmoothError(f"{what}: status {rc}: {msg}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.asm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, option: int, value: int) -> None:
+        self._check(self._L.asm_set_option(self._h, option, int(value)), "asm_set_option")
+
+    def synchronize(self) -> None:
+        self._check(self._L.asm_synchronize(self._h), "asm_synchronize")
+
+    # -- the path ---------------------------------------------------------
+    def set_particles(self, particles: np.ndarray, np_local: int, boxsize: float) -> None:
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        self._check(self._L.asm_set_particles(self._h, _ptr(p), len(p), int(np_local), float(boxsize)),
+                    "asm_set_particles")
+        self.n, self.np_local, self.boxsize = len(p), int(np_local), float(boxsize)
+
+    def set_particles_device(self, dev_ptr: int, n: int, np_local: int, boxsize: float) -> None:
+        self._check(self._L.asm_set_particles_device(self._h, C.c_void_p(dev_ptr), int(n), int(np_local),
+                                                     float(boxsize)), "asm_set_particles_device")
+        self.n, self.np_local, self.boxsize = int(n), int(np_local), float(boxsize)
+
+    def construct(self, left: float, right: float) -> int:
+        nn = C.c_int64()
+        self._check(self._L.asm_tree_construct(self._h, float(left), float(right), C.byref(nn)),
+                    "asm_tree_construct")
+        return nn.value
+
+    def neighbor_search(self, k: int = 32) -> None:
+        self._check(self._L.asm_neighbor_search(self._h, int(k)), "asm_neighbor_search")
+
+    def mark_halo_particles(self, halos: np.ndarray, left: float, right: float) -> tuple[int, float]:
+        h = np.ascontiguousarray(halos, dtype=HALO_DTYPE)
+        nph, mh = C.c_int64(), C.c_double()
+        self._check(self._L.asm_mark_halo_particles(self._h, _ptr(h) if len(h) else None, len(h),
+                                                    float(left), float(right), C.byref(nph), C.byref(mh)),
+                    "asm_mark_halo_particles")
+        return nph.value, mh.value
+
+    def assign_on_mesh(self, nc: int, sign: int = -1) -> None:
+        self._check(self._L.asm_assign_on_mesh(self._h, int(nc), int(sign)), "asm_assign_on_mesh")
+
+    def assign_on_meshes(self, ncs: Sequence[int], sign: int = -1) -> None:
+        arr = (C.c_int * len(ncs))(*[int(v) for v in ncs])
+        self._check(self._L.asm_assign_on_meshes(self._h, arr, len(ncs), int(sign)), "asm_assign_on_meshes")
+
+    def assign_on_global_mesh(self, nc: int, sign: int = -1, coord=(0, 0, 0)) -> None:
+        arr = (C.c_int * 3)(*[int(v) for v in coord])
+        self._check(self._L.asm_assign_on_global_mesh(self._h, int(nc), int(sign), arr),
+                    "asm_assign_on_global_mesh")
+
+    def mesh_total(self, nc: int) -> tuple[float, float]:
+        out = (C.c_double * 2)()
+        self._check(self._L.asm_mesh_total(self._h, int(nc), out), "asm_mesh_total")
+        return out[0], out[1]
+
+    def mesh_stats(self, nc: int) -> tuple[float, float, float, float]:
+        out = (C.c_double * 4)()
+        self._check(self._L.asm_mesh_stats(self._h, int(nc), out), "asm_mesh_stats")
+        return tuple(out)  # type: ignore[return-value]
+
+    def normalize_mesh(self, nc: int) -> None:
+        self._check(self._L.asm_normalize_mesh(self._h, int(nc)), "asm_normalize_mesh")
+
+    def get_mesh(self, nc: int) -> np.ndarray:
+        m = np.empty(6 * nc ** 3, dtype=np.float32)
+        self._check(self._L.asm_get_mesh(self._h, int(nc), _ptr(m)), "asm_get_mesh")
+        return m
+
+    def mesh_device_ptr(self, nc: int) -> int:
+        return int(self._L.asm_mesh_device_ptr(self._h, int(nc)) or 0)
+
+    def get_particles(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=PARTICLE_DTYPE)
+        self._check(self._L.asm_get_particles(self._h, _ptr(out)), "asm_get_particles")
+        return out
+
+    def get_rk_dens(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        rk = np.empty(self.n, np.float32)
+        dens = np.empty(self.n, np.float32)
+        igrp = np.empty(self.n, np.int32)
+        self._check(self._L.asm_get_rk_dens(self._h, _ptr(rk), _ptr(dens), _ptr(igrp)), "asm_get_rk_dens")
+        return rk, dens, igrp
+
+    def get_neighbors(self) -> np.ndarray:
+        nbr = np.empty((self.n, K_SLOTS), np.int32)
+        self._check(self._L.asm_get_neighbors(self._h, _ptr(nbr)), "asm_get_neighbors")
+        return nbr
+
+    # -- instrumentation -------------------------------------------------
+    def timers(self) -> dict[str, tuple[float, int]]:
+        out = {}
+        for i, name in enumerate(timer_names()):
+            ms, n = C.c_double(), C.c_int64()
+            self._check(self._L.asm_get_timer(self._h, i, C.byref(ms), C.byref(n)), "asm_get_timer")
+            out[name] = (ms.value, n.value)
+        return out
+
+    def reset_timers(self) -> None:
+        self._check(self._L.asm_reset_timers(self._h), "asm_reset_timers")
+
+    def search_stats(self) -> dict[str, int]:
+        out = (C.c_uint64 * 4)()
+        self._check(self._L.asm_get_search_stats(self._h, out), "asm_get_search_stats")
+        return dict(zip(["node_tests", "leaf_scans", "dist_evals", "inserts"], list(out)))
+
+
+# -- host steps either side of the path ------------------------------------
+
+def particles_from_xv(xv: np.ndarray) -> np.ndarray:
+    """Particle records from an (n, 6) x/v array (dens = rk = igrp = 0)."""
+    xv = np.asarray(xv, np.float32).reshape(-1, 6)
+    p = np.zeros(len(xv), PARTICLE_DTYPE)
+    p["x"] = xv[:, :3]
+    p["v"] = xv[:, 3:]
+    return p
+
+
+def build_buffered_set(xv: np.ndarray, boxsize: float, bf: float) -> tuple[np.ndarray, int]:
+    """Single periodic rank: exchange_buffer images + remove_extra_particles."""
+    xv = np.ascontiguousarray(xv, np.float32).reshape(-1, 6)
+    cap = int(len(xv) * (1 + 2 * bf) ** 3 * 1.2) + 1024
+    out = np.empty(cap, PARTICLE_DTYPE)
+    nl = C.c_int64()
+    n = lib().asm_build_buffered_set(_ptr(xv), len(xv), float(boxsize), float(bf), _ptr(out), cap, C.byref(nl))
+    if n < 0:
+        raise AsmoothError(f"asm_build_buffered_set failed with status {n}")
+    return out[:n].copy(), nl.value
+
+
+def build_node_share(xv: np.ndarray, box_total: float, nn: int, coord, bf: float) -> tuple[np.ndarray, int]:
+    """Node `coord` of an nn^3 decomposition of a periodic box (node coordinates)."""
+    xv = np.ascontiguousarray(xv, np.float32).reshape(-1, 6)
+    cap = int(len(xv) / nn ** 3 * (1 + 2 * bf) ** 3 * 1.5) + 4096
+    out = np.empty(cap, PARTICLE_DTYPE)
+    nl = C.c_int64()
+    c3 = (C.c_int * 3)(*[int(v) for v in coord])
+    n = lib().asm_build_node_share(_ptr(xv), len(xv), float(box_total), int(nn), c3, float(bf), _ptr(out), cap,
+                                   C.byref(nl))
+    if n < 0:
+        raise AsmoothError(f"asm_build_node_share failed with status {n}")
+    return out[:n].copy(), nl.value
+
+
+def read_xv(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """cubep3m xv file -> ((n, 6) float32, header[11])."""
+    L = lib()
+    buf, n = C.c_void_p(), C.c_int64()
+    hdr = np.zeros(11, np.float32)
+    rc = L.asm_read_xv(path.encode(), C.byref(buf), C.byref(n), _ptr(hdr))
+    if rc < 0:
+        raise AsmoothError(f"asm_read_xv({path}) failed with status {rc}")
+    try:
+        arr = np.ctypeslib.as_array(C.cast(buf, C.POINTER(C.c_float)), shape=(n.value * 6,)).copy()
+    finally:
+        L.asm_free(buf)
+    return arr.reshape(-1, 6), hdr
+
+
+def write_xv(path: str, xv: np.ndarray, header=None) -> None:
+    xv = np.ascontiguousarray(xv, np.float32).reshape(-1, 6)
+    hdr = np.zeros(11, np.float32) if header is None else np.ascontiguousarray(header, np.float32)
+    rc = lib().asm_write_xv(path.encode(), _ptr(xv), len(xv), _ptr(hdr))
+    if rc < 0:
+        raise AsmoothError(f"asm_write_xv({path}) failed with status {rc}")
+
+
+def write_mesh_separate(base: str, inode: int, mesh: np.ndarray, nc: int) -> None:
+    m = np.ascontiguousarray(mesh, np.float32)
+    rc = lib().asm_write_mesh_separate(base.encode(), int(inode), _ptr(m), int(nc))
+    if rc < 0:
+        raise AsmoothError(f"asm_write_mesh_separate({base}) failed with status {rc}")

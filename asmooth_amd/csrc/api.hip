@@ -1,0 +1,668 @@
+// C ABI of the device path: context, device buffers, call sequencing, timers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/asmooth.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace asm_hip;
+
+namespace {
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n)
+    {
+        if (n <= cap && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t bytes = (n > 0 ? n : 1) * sizeof(T);
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), bytes);
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+enum TimerId {
+    T_SORT = 0,
+    T_TREE,
+    T_KNN,
+    T_DEPOSIT_LIST,
+    T_DEPOSIT_TILES,
+    T_REDUCE,
+    T_NORMALIZE,
+    T_HALO,
+    T_COUNT
+};
+const char* kTimerNames[T_COUNT] = {"sort", "tree", "knn_dens", "deposit_lists", "deposit_tiles",
+                                    "mesh_reduce", "normalize", "mark_halo"};
+
+struct MeshDev {
+    DevBuf<float> mesh;
+};
+
+}  // namespace
+
+struct asm_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int keep_neighbors = 0;
+    int profile = 0;
+    int leaf_size = 32;
+    int tile = 8;
+
+    // particle state
+    int64_t n = 0;
+    int64_t np_local = 0;
+    float boxsize = 0.0f;
+    bool have_particles = false;
+    bool have_tree = false;
+    bool have_search = false;
+    DevBuf<ParticleRec> input;   // caller order
+    // sort / tree
+    DevBuf<uint32_t> keys, vals, keys_tmp, vals_tmp, hist;
+    DevBuf<float4> spos, sattr;
+    DevBuf<int32_t> inv_perm;
+    DevBuf<uint32_t> perm_sorted;  // sorted position -> original index (points into vals or vals_tmp)
+    uint32_t* perm = nullptr;
+    DevBuf<Box> nodes;
+    int64_t nleaf = 0, nleaf_pad = 0;
+    // search results (sorted order)
+    DevBuf<float> rk, dens;
+    DevBuf<double> dens64;
+    DevBuf<int32_t> nbr;
+    DevBuf<KnnStats> kstats;
+    KnnStats last_stats{};
+    // deposition
+    DevBuf<uint32_t> tile_cnt, tile_start, tile_cur, list, scan_total;
+    DevBuf<double> red_partial, red_out;
+    std::map<int, MeshDev> meshes;
+    // halos
+    DevBuf<HaloRec> halos;
+    DevBuf<unsigned long long> halo_count;
+    // timers
+    struct Pending {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+    double timer_ms[T_COUNT] = {0};
+    int64_t timer_n[T_COUNT] = {0};
+};
+
+namespace {
+
+int fail(asm_context* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPC(expr)                                                                                  \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess)                                                                       \
+            return fail(ctx, ASM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),    \
+                        __FILE__, __LINE__);                                                        \
+    } while (0)
+
+hipEvent_t get_event(asm_context* c)
+{
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Scoped timer: records an event pair around a stage when profiling is on.
+struct StageTimer {
+    asm_context* c;
+    int id;
+    hipEvent_t a = nullptr;
+    StageTimer(asm_context* c_, int id_) : c(c_), id(id_)
+    {
+        if (c->profile) {
+            a = get_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~StageTimer()
+    {
+        if (c->profile && a) {
+            hipEvent_t b = get_event(c);
+            if (b) {
+                (void)hipEventRecord(b, c->stream);
+                c->pending.push_back({id, a, b});
+            }
+        }
+    }
+};
+
+hipError_t resolve_timers(asm_context* c)
+{
+    if (c->pending.empty()) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    for (auto& p : c->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->timer_ms[p.id] += ms;
+            c->timer_n[p.id] += 1;
+        }
+        c->free_events.push_back(p.a);
+        c->free_events.push_back(p.b);
+    }
+    c->pending.clear();
+    return hipSuccess;
+}
+
+int64_t next_pow2(int64_t v)
+{
+    int64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+int load_common(asm_context* ctx, int64_t n, int64_t np_local, float boxsize)
+{
+    if (n < kSlots) return fail(ctx, ASM_ERR_ARG, "need at least %d particles, got %lld", kSlots, (long long)n);
+    if (n > 0x7fffffffLL) return fail(ctx, ASM_ERR_CAPACITY, "np_with_buffers %lld exceeds 2^31-1", (long long)n);
+    if (np_local < 0 || np_local > n) return fail(ctx, ASM_ERR_ARG, "np_local %lld out of range", (long long)np_local);
+    if (!(boxsize > 0.0f)) return fail(ctx, ASM_ERR_ARG, "boxsize must be positive");
+    ctx->n = n;
+    ctx->np_local = np_local;
+    ctx->boxsize = boxsize;
+    ctx->have_particles = true;
+    ctx->have_tree = false;
+    ctx->have_search = false;
+    return ASM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* asm_version(void)
+{
+    return "asmooth_amd 0.1 (gfx950)";
+}
+
+int asm_create(asm_context** out, int device)
+{
+    if (!out) return ASM_ERR_ARG;
+    *out = nullptr;
+    asm_context* ctx = new asm_context();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return ASM_ERR_HIP;
+    }
+    *out = ctx;
+    return ASM_OK;
+}
+
+void asm_destroy(asm_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* asm_last_error(const asm_context* ctx)
+{
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+int asm_set_option(asm_context* ctx, int option, int value)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    switch (option) {
+    case ASM_OPT_KEEP_NEIGHBORS: ctx->keep_neighbors = value ? 1 : 0; return ASM_OK;
+    case ASM_OPT_PROFILE: ctx->profile = value ? 1 : 0; return ASM_OK;
+    case ASM_OPT_LEAF_SIZE:
+        if (value < 8 || value > kWave) return fail(ctx, ASM_ERR_ARG, "leaf size %d outside 8..64", value);
+        ctx->leaf_size = value;
+        ctx->have_tree = false;
+        ctx->have_search = false;
+        return ASM_OK;
+    case ASM_OPT_TILE:
+        if (value < 1 || value > 8) return fail(ctx, ASM_ERR_ARG, "tile %d outside 1..8", value);
+        ctx->tile = value;
+        return ASM_OK;
+    default: return fail(ctx, ASM_ERR_ARG, "unknown option %d", option);
+    }
+}
+
+int asm_synchronize(asm_context* ctx)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(resolve_timers(ctx));
+    return ASM_OK;
+}
+
+int asm_set_particles(asm_context* ctx, const asm_particle* particles, int64_t n, int64_t np_local,
+                      float boxsize)
+{
+    if (!ctx || !particles) return fail(ctx, ASM_ERR_ARG, "null argument");
+    int rc = load_common(ctx, n, np_local, boxsize);
+    if (rc) return rc;
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(ctx->input.ensure(static_cast<size_t>(n)));
+    HIPC(hipMemcpyAsync(ctx->input.p, particles, sizeof(ParticleRec) * static_cast<size_t>(n),
+                        hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return ASM_OK;
+}
+
+int asm_set_particles_device(asm_context* ctx, const asm_particle* d_particles, int64_t n, int64_t np_local,
+                             float boxsize)
+{
+    if (!ctx || !d_particles) return fail(ctx, ASM_ERR_ARG, "null argument");
+    int rc = load_common(ctx, n, np_local, boxsize);
+    if (rc) return rc;
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(ctx->input.ensure(static_cast<size_t>(n)));
+    HIPC(hipMemcpyAsync(ctx->input.p, d_particles, sizeof(ParticleRec) * static_cast<size_t>(n),
+                        hipMemcpyDeviceToDevice, ctx->stream));
+    return ASM_OK;
+}
+
+int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    if (!ctx->have_particles) return fail(ctx, ASM_ERR_STATE, "asm_tree_construct before asm_set_particles");
+    if (!(right > left)) return fail(ctx, ASM_ERR_ARG, "tree cube [%g, %g] is empty", left, right);
+    HIPC(hipSetDevice(ctx->device));
+    const int64_t n = ctx->n;
+    hipStream_t s = ctx->stream;
+    HIPC(ctx->keys.ensure(n));
+    HIPC(ctx->vals.ensure(n));
+    HIPC(ctx->keys_tmp.ensure(n));
+    HIPC(ctx->vals_tmp.ensure(n));
+    HIPC(ctx->hist.ensure(static_cast<size_t>(radix_hist_entries(n))));
+    HIPC(ctx->spos.ensure(n));
+    HIPC(ctx->sattr.ensure(n));
+    HIPC(ctx->inv_perm.ensure(n));
+    {
+        StageTimer t(ctx, T_SORT);
+        HIPC(launch_morton_keys(ctx->input.p, n, left, right, ctx->keys.p, ctx->vals.p, s));
+        bool in_tmp = false;
+        HIPC(launch_radix_sort(ctx->keys.p, ctx->vals.p, ctx->keys_tmp.p, ctx->vals_tmp.p, n, 30, ctx->hist.p, s,
+                               &in_tmp));
+        ctx->perm = in_tmp ? ctx->vals_tmp.p : ctx->vals.p;
+    }
+    ctx->nleaf = (n + ctx->leaf_size - 1) / ctx->leaf_size;
+    ctx->nleaf_pad = next_pow2(ctx->nleaf);
+    HIPC(ctx->nodes.ensure(static_cast<size_t>(2 * ctx->nleaf_pad)));
+    {
+        StageTimer t(ctx, T_TREE);
+        HIPC(launch_gather_sorted(ctx->input.p, ctx->perm, n, ctx->spos.p, ctx->sattr.p, ctx->inv_perm.p, s));
+        HIPC(launch_build_tree(ctx->spos.p, n, ctx->leaf_size, ctx->nleaf, ctx->nleaf_pad, ctx->nodes.p, s));
+    }
+    ctx->have_tree = true;
+    ctx->have_search = false;
+    if (nnode) *nnode = 2 * ctx->nleaf_pad - 1;
+    return ASM_OK;
+}
+
+int asm_neighbor_search(asm_context* ctx, int k)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    if (k != kSlots - 1) return fail(ctx, ASM_ERR_ARG, "k must be %d (33 slots incl. self), got %d", kSlots - 1, k);
+    if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "asm_neighbor_search before asm_tree_construct");
+    HIPC(hipSetDevice(ctx->device));
+    const int64_t n = ctx->n;
+    hipStream_t s = ctx->stream;
+    HIPC(ctx->rk.ensure(n));
+    HIPC(ctx->dens.ensure(n));
+    HIPC(ctx->dens64.ensure(n));
+    HIPC(ctx->kstats.ensure(1));
+    int32_t* nbr = nullptr;
+    if (ctx->keep_neighbors) {
+        HIPC(ctx->nbr.ensure(static_cast<size_t>(n) * kSlots));
+        nbr = ctx->nbr.p;
+    }
+    {
+        StageTimer t(ctx, T_KNN);
+        HIPC(launch_fill_zero_f64(ctx->dens64.p, n, s));
+        HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->kstats.p), sizeof(KnnStats) / 4, s));
+        HIPC(launch_knn_dens(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->inv_perm.p, ctx->rk.p,
+                             ctx->dens64.p, nbr, ctx->kstats.p, s));
+        HIPC(launch_dens_finalize(ctx->dens64.p, n, ctx->dens.p, s));
+    }
+    HIPC(hipMemcpyAsync(&ctx->last_stats, ctx->kstats.p, sizeof(KnnStats), hipMemcpyDeviceToHost, s));
+    ctx->have_search = true;
+    return ASM_OK;
+}
+
+int asm_mark_halo_particles(asm_context* ctx, const asm_halo* halos, int64_t n_halo, float left, float right,
+                            int64_t* np_halo, double* m_halo)
+{
+    if (!ctx || (n_halo > 0 && !halos)) return fail(ctx, ASM_ERR_ARG, "null argument");
+    if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "asm_mark_halo_particles before asm_tree_construct");
+    HIPC(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    HIPC(ctx->halo_count.ensure(1));
+    HIPC(hipMemsetAsync(ctx->halo_count.p, 0, sizeof(unsigned long long), s));
+    double mass = 0.0;
+    for (int64_t i = 0; i < n_halo; ++i) {
+        const asm_halo& h = halos[i];
+        if (left < h.x[0] && h.x[0] < right && left < h.x[1] && h.x[1] < right && left < h.x[2] && h.x[2] < right)
+            mass += static_cast<double>(h.mass / 8.0f);
+    }
+    if (n_halo > 0) {
+        HIPC(ctx->halos.ensure(static_cast<size_t>(n_halo)));
+        HIPC(hipMemcpyAsync(ctx->halos.p, halos, sizeof(HaloRec) * static_cast<size_t>(n_halo),
+                            hipMemcpyHostToDevice, s));
+        StageTimer t(ctx, T_HALO);
+        HIPC(launch_mark_halo(ctx->spos.p, ctx->sattr.p, ctx->n, ctx->nodes.p, ctx->nleaf_pad, ctx->halos.p, n_halo,
+                              left, right, ctx->halo_count.p, s));
+    }
+    unsigned long long cnt = 0;
+    HIPC(hipMemcpyAsync(&cnt, ctx->halo_count.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (np_halo) *np_halo = static_cast<int64_t>(cnt);
+    if (m_halo) *m_halo = mass;
+    return ASM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coord)
+{
+    if (nc <= 0 || nc > 2048) return fail(ctx, ASM_ERR_ARG, "nc %d outside 1..2048", nc);
+    if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "mesh assignment before asm_neighbor_search");
+    HIPC(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int64_t nmesh = static_cast<int64_t>(nc) * nc * nc;
+    MeshDev& m = ctx->meshes[nc];
+    HIPC(m.mesh.ensure(static_cast<size_t>(6 * nmesh)));
+
+    DepositArgs a{};
+    a.spos = ctx->spos.p;
+    a.sattr = ctx->sattr.p;
+    a.rk = ctx->rk.p;
+    a.dens = ctx->dens.p;
+    a.n = ctx->n;
+    a.boxsize = ctx->boxsize;
+    a.nc = nc;
+    a.sign = sign;
+    a.tile = ctx->tile;
+    a.ntx = (nc + ctx->tile - 1) / ctx->tile;
+    a.periodic = periodic;
+    a.coord[0] = coord ? coord[0] : 0;
+    a.coord[1] = coord ? coord[1] : 0;
+    a.coord[2] = coord ? coord[2] : 0;
+    const int64_t ntiles = static_cast<int64_t>(a.ntx) * a.ntx * a.ntx;
+    HIPC(ctx->tile_cnt.ensure(ntiles));
+    HIPC(ctx->tile_start.ensure(ntiles));
+    HIPC(ctx->tile_cur.ensure(ntiles));
+    HIPC(ctx->scan_total.ensure(1));
+    uint32_t total = 0;
+    {
+        StageTimer t(ctx, T_DEPOSIT_LIST);
+        HIPC(launch_fill_zero_u32(ctx->tile_cnt.p, ntiles, s));
+        HIPC(launch_deposit_count(a, ctx->tile_cnt.p, s));
+        HIPC(hipMemcpyAsync(ctx->tile_start.p, ctx->tile_cnt.p, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToDevice, s));
+        HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, s));
+    }
+    HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    HIPC(ctx->list.ensure(total > 0 ? total : 1));
+    {
+        StageTimer t(ctx, T_DEPOSIT_LIST);
+        HIPC(launch_fill_zero_u32(ctx->tile_cur.p, ntiles, s));
+        HIPC(launch_deposit_fill(a, ctx->tile_start.p, ctx->tile_cur.p, ctx->list.p, s));
+    }
+    {
+        StageTimer t(ctx, T_DEPOSIT_TILES);
+        HIPC(launch_deposit_tiles(a, ctx->tile_start.p, ctx->tile_cnt.p, ctx->list.p, m.mesh.p, 0, s));
+    }
+    return ASM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int asm_assign_on_mesh(asm_context* ctx, int nc, int sign)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    return deposit_one(ctx, nc, sign, 0, nullptr);
+}
+
+int asm_assign_on_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
+{
+    if (!ctx || !ncs || n_nc <= 0) return fail(ctx, ASM_ERR_ARG, "empty nc list");
+    for (int i = 0; i < n_nc; ++i) {
+        int rc = deposit_one(ctx, ncs[i], sign, 0, nullptr);
+        if (rc) return rc;
+    }
+    return ASM_OK;
+}
+
+int asm_assign_on_global_mesh(asm_context* ctx, int nc, int sign, const int coord[3])
+{
+    if (!ctx || !coord) return fail(ctx, ASM_ERR_ARG, "null argument");
+    return deposit_one(ctx, nc, sign, 1, coord);
+}
+
+int asm_mesh_stats(asm_context* ctx, int nc, double out[4])
+{
+    if (!ctx || !out) return ASM_ERR_ARG;
+    auto it = ctx->meshes.find(nc);
+    if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(ctx->red_partial.ensure(4 * 1024));
+    HIPC(ctx->red_out.ensure(4));
+    {
+        StageTimer t(ctx, T_REDUCE);
+        HIPC(launch_mesh_reduce(it->second.mesh.p, nc, ctx->red_partial.p, ctx->red_out.p, ctx->stream));
+    }
+    HIPC(hipMemcpyAsync(out, ctx->red_out.p, sizeof(double) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return ASM_OK;
+}
+
+int asm_mesh_total(asm_context* ctx, int nc, double totals[2])
+{
+    double out[4];
+    int rc = asm_mesh_stats(ctx, nc, out);
+    if (rc) return rc;
+    totals[0] = out[0];
+    totals[1] = out[1];
+    return ASM_OK;
+}
+
+int asm_normalize_mesh(asm_context* ctx, int nc)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    auto it = ctx->meshes.find(nc);
+    if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
+    HIPC(hipSetDevice(ctx->device));
+    // coarse_mesh.cpp:151: vol = pow(boxsize/nc, 3) in FP32.
+    const float vol = std::pow(ctx->boxsize / static_cast<float>(nc), 3.0f);
+    StageTimer t(ctx, T_NORMALIZE);
+    HIPC(launch_normalize(it->second.mesh.p, nc, vol, ctx->stream));
+    return ASM_OK;
+}
+
+int asm_get_mesh(asm_context* ctx, int nc, float* mesh)
+{
+    if (!ctx || !mesh) return ASM_ERR_ARG;
+    auto it = ctx->meshes.find(nc);
+    if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
+    HIPC(hipSetDevice(ctx->device));
+    const size_t nmesh = static_cast<size_t>(nc) * nc * nc;
+    HIPC(hipMemcpyAsync(mesh, it->second.mesh.p, sizeof(float) * 6 * nmesh, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return ASM_OK;
+}
+
+const float* asm_mesh_device_ptr(asm_context* ctx, int nc)
+{
+    if (!ctx) return nullptr;
+    auto it = ctx->meshes.find(nc);
+    return it == ctx->meshes.end() ? nullptr : it->second.mesh.p;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Sorted-order per-particle values back to the caller's order.
+__global__ void unsort_kernel(const float4* __restrict__ spos, const float4* __restrict__ sattr,
+                              const float* __restrict__ rk, const float* __restrict__ dens, int64_t n,
+                              float* __restrict__ rk_o, float* __restrict__ dens_o, int32_t* __restrict__ igrp_o)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = __float_as_uint(spos[i].w);
+    rk_o[o] = rk ? rk[i] : 0.0f;
+    dens_o[o] = dens ? dens[i] : 0.0f;
+    igrp_o[o] = __float_as_int(sattr[i].w);
+}
+
+}  // namespace
+
+extern "C" {
+
+int asm_get_rk_dens(asm_context* ctx, float* rk, float* dens, int32_t* igrp)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "no results before asm_tree_construct");
+    HIPC(hipSetDevice(ctx->device));
+    const int64_t n = ctx->n;
+    hipStream_t s = ctx->stream;
+    DevBuf<float> r, d;
+    DevBuf<int32_t> g;
+    HIPC(r.ensure(n));
+    HIPC(d.ensure(n));
+    HIPC(g.ensure(n));
+    unsort_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(
+        ctx->spos.p, ctx->sattr.p, ctx->have_search ? ctx->rk.p : nullptr,
+        ctx->have_search ? ctx->dens.p : nullptr, n, r.p, d.p, g.p);
+    HIPC(hipGetLastError());
+    if (rk) HIPC(hipMemcpyAsync(rk, r.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (dens) HIPC(hipMemcpyAsync(dens, d.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (igrp) HIPC(hipMemcpyAsync(igrp, g.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return ASM_OK;
+}
+
+int asm_get_particles(asm_context* ctx, asm_particle* particles)
+{
+    if (!ctx || !particles) return ASM_ERR_ARG;
+    const int64_t n = ctx->n;
+    std::vector<float> rk(n), dens(n);
+    std::vector<int32_t> igrp(n);
+    int rc = asm_get_rk_dens(ctx, rk.data(), dens.data(), igrp.data());
+    if (rc) return rc;
+    HIPC(hipMemcpy(particles, ctx->input.p, sizeof(ParticleRec) * static_cast<size_t>(n), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) {
+        particles[i].rk = rk[i];
+        particles[i].dens = dens[i];
+        particles[i].igrp = igrp[i];
+    }
+    return ASM_OK;
+}
+
+int asm_get_neighbors(asm_context* ctx, int32_t* nbr)
+{
+    if (!ctx || !nbr) return ASM_ERR_ARG;
+    if (!ctx->have_search || !ctx->keep_neighbors || !ctx->nbr.p)
+        return fail(ctx, ASM_ERR_STATE, "neighbour lists not kept (set ASM_OPT_KEEP_NEIGHBORS before the search)");
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(hipMemcpyAsync(nbr, ctx->nbr.p, sizeof(int32_t) * kSlots * static_cast<size_t>(ctx->n),
+                        hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return ASM_OK;
+}
+
+int asm_timer_count(void)
+{
+    return T_COUNT;
+}
+
+const char* asm_timer_name(int i)
+{
+    return (i >= 0 && i < T_COUNT) ? kTimerNames[i] : "";
+}
+
+int asm_get_timer(asm_context* ctx, int i, double* total_ms, int64_t* launches)
+{
+    if (!ctx || i < 0 || i >= T_COUNT) return ASM_ERR_ARG;
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(resolve_timers(ctx));
+    if (total_ms) *total_ms = ctx->timer_ms[i];
+    if (launches) *launches = ctx->timer_n[i];
+    return ASM_OK;
+}
+
+int asm_reset_timers(asm_context* ctx)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(resolve_timers(ctx));
+    for (int i = 0; i < T_COUNT; ++i) {
+        ctx->timer_ms[i] = 0.0;
+        ctx->timer_n[i] = 0;
+    }
+    return ASM_OK;
+}
+
+int asm_get_search_stats(asm_context* ctx, uint64_t out[4])
+{
+    if (!ctx || !out) return ASM_ERR_ARG;
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    out[0] = ctx->last_stats.node_tests;
+    out[1] = ctx->last_stats.leaf_scans;
+    out[2] = ctx->last_stats.dist_evals;
+    out[3] = ctx->last_stats.inserts;
+    return ASM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,401 @@
+// Mesh deposition of the adaptive tent kernel, mesh reductions, normalisation.
+//
+// Replaces clear_mesh / assign_on_mesh / mesh_total (coarse_mesh.cpp:24-127),
+// the periodic variant assign_on_global_mesh (global_mesh.cpp:25-123) and the
+// normalisation in write_mesh_separate (coarse_mesh.cpp:150-171).
+//
+// The mesh is cut into cubic tiles of `tile` cells.  Every particle is listed
+// in each tile its cell range overlaps (count, scan, fill).  One workgroup then
+// owns one tile: it accumulates every (particle, cell) contribution into FP64
+// accumulators in LDS with native ds_add_f64, and flushes the tile once with
+// plain stores - no global atomics, every cell written exactly once.  Each
+// contribution is the reference's FP32 term, bit for bit (cell range
+// coarse_mesh.cpp:72-75, weights :80-93, products :95-122); only the sums are
+// carried in FP64, so each stored cell is the FP32 rounding of an FP64 sum.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace asm_hip {
+namespace {
+
+constexpr int kTileMax = 8;                 // cells per tile side (LDS sized for 8^3)
+constexpr int kTileCells = kTileMax * kTileMax * kTileMax;
+constexpr int kDepThreads = 256;
+
+// coarse_mesh.cpp:10-15.
+__device__ __forceinline__ float tent_f(float x)
+{
+    return x >= 1.0f ? 0.5f : x - 0.5f * x * x;
+}
+
+// coarse_mesh.cpp:17-22: integral of the unit tent over [x1, x2].
+__device__ __forceinline__ float ker_integ(float x1, float x2)
+{
+    const float fac1 = (x1 > 0.0f) ? 1.0f : -1.0f;
+    const float fac2 = (x2 > 0.0f) ? 1.0f : -1.0f;
+    return fac2 * tent_f(fac2 * x2) - fac1 * tent_f(fac1 * x1);
+}
+
+// Cell range [lo, hi) of one axis.  Node mesh: coarse_mesh.cpp:72-75
+// (truncation toward zero, clipped to [0, nc)).  Periodic variant:
+// global_mesh.cpp:66-69 (floor / ceil, unclipped, wrapped at use).
+__device__ __forceinline__ void axis_range(float x, float h, float boxsize, float ncf, int nc,
+                                           int periodic, int& lo, int& hi)
+{
+    if (!periodic) {
+        const int l = static_cast<int>((x - h) / boxsize * ncf);
+        const int r = 1 + static_cast<int>((x + h) / boxsize * ncf);
+        lo = l > 0 ? l : 0;
+        hi = r < nc ? r : nc;
+    } else {
+        lo = static_cast<int>(floorf((x - h) / boxsize * ncf));
+        hi = 1 + static_cast<int>(ceilf((x + h) / boxsize * ncf));
+    }
+}
+
+struct PartRange {
+    float x[3];
+    float h;
+    int lo[3];
+    int hi[3];
+    bool active;
+};
+
+// Position as deposited and cell ranges for particle i.
+__device__ __forceinline__ PartRange part_range(const DepositArgs& a, int64_t i)
+{
+    PartRange r;
+    const float4 p = a.spos[i];
+    r.h = a.rk[i];
+    r.x[0] = p.x;
+    r.x[1] = p.y;
+    r.x[2] = p.z;
+    r.active = true;
+    if (a.periodic) {
+        // Local particles only, shifted by the node offset (global_mesh.cpp:43-56).
+        if (p.x < 0.0f || p.x >= a.boxsize || p.y < 0.0f || p.y >= a.boxsize || p.z < 0.0f ||
+            p.z >= a.boxsize)
+            r.active = false;
+        r.x[0] = p.x + a.boxsize * static_cast<float>(a.coord[0]);
+        r.x[1] = p.y + a.boxsize * static_cast<float>(a.coord[1]);
+        r.x[2] = p.z + a.boxsize * static_cast<float>(a.coord[2]);
+    }
+    const float ncf = static_cast<float>(a.nc);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        axis_range(r.x[d], r.h, a.boxsize, ncf, a.nc, a.periodic, r.lo[d], r.hi[d]);
+        if (r.lo[d] >= r.hi[d]) r.active = false;
+    }
+    return r;
+}
+
+__device__ __forceinline__ int floor_div(int a, int b)
+{
+    return (a >= 0) ? a / b : -((-a + b - 1) / b);
+}
+
+// Tile range of one axis for cell range [lo, hi).  Periodic ranges can run
+// past [0, nc); they are split by wrapping, so a tile index is taken modulo
+// the tile count and a range is limited to at most one full turn.
+template <typename F>
+__device__ __forceinline__ void for_each_tile(const DepositArgs& a, const PartRange& r, F&& f)
+{
+    int t0[3], t1[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        t0[d] = floor_div(r.lo[d], a.tile);
+        t1[d] = floor_div(r.hi[d] - 1, a.tile);
+        if (a.periodic && t1[d] - t0[d] >= a.ntx) t1[d] = t0[d] + a.ntx - 1;
+    }
+    for (int tz = t0[2]; tz <= t1[2]; ++tz) {
+        const int wz = ((tz % a.ntx) + a.ntx) % a.ntx;
+        for (int ty = t0[1]; ty <= t1[1]; ++ty) {
+            const int wy = ((ty % a.ntx) + a.ntx) % a.ntx;
+            for (int tx = t0[0]; tx <= t1[0]; ++tx) {
+                const int wx = ((tx % a.ntx) + a.ntx) % a.ntx;
+                f((static_cast<int64_t>(wz) * a.ntx + wy) * a.ntx + wx);
+            }
+        }
+    }
+}
+
+__global__ void deposit_count_kernel(DepositArgs a, uint32_t* __restrict__ tile_cnt)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const PartRange r = part_range(a, i);
+    if (!r.active) return;
+    for_each_tile(a, r, [&](int64_t t) { atomicAdd(&tile_cnt[t], 1u); });
+}
+
+__global__ void deposit_fill_kernel(DepositArgs a, const uint32_t* __restrict__ tile_start,
+                                    uint32_t* __restrict__ tile_cur, uint32_t* __restrict__ list)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const PartRange r = part_range(a, i);
+    if (!r.active) return;
+    for_each_tile(a, r, [&](int64_t t) {
+        const uint32_t slot = atomicAdd(&tile_cur[t], 1u);
+        list[tile_start[t] + slot] = static_cast<uint32_t>(i);
+    });
+}
+
+// One workgroup per tile: FP64 LDS accumulation, then one flush of the tile.
+__global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
+    DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
+    const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
+{
+    __shared__ double acc[6][kTileCells];
+    __shared__ float wxs[kDepThreads][kTileMax + 1];
+    const int tid = threadIdx.x;
+    const int T = a.tile;
+    const int64_t tile = blockIdx.x;
+    const int tx = static_cast<int>(tile % a.ntx);
+    const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
+    const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
+    const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
+    const int ncell = T * T * T;
+    for (int c = tid; c < 6 * kTileCells; c += kDepThreads) (&acc[0][0])[c] = 0.0;
+    __syncthreads();
+
+    const float dxc = a.boxsize / static_cast<float>(a.nc);
+    const uint32_t begin = tile_start[tile];
+    const uint32_t count = tile_cnt[tile];
+    for (uint32_t li = tid; li < count; li += kDepThreads) {
+        const int64_t i = list[begin + li];
+        const PartRange r = part_range(a, i);
+        const float hinv = 1.0f / r.h;
+        const float ds = dxc * hinv;
+        const float4 at = a.sattr[i];
+        const float dens = a.dens[i];
+        const int igrp = __float_as_int(at.w);
+        const bool igm = (a.sign * igrp <= 0);
+        // Periodic variant: the unwrapped cell range may run past either end of
+        // [0, nc); cell i lands on tile cell i - k*nc - c0 for the shift k that
+        // puts it inside the tile (k in {-1, 0, 1} since h < boxsize).
+        const int kr = a.periodic ? 1 : 0;
+        for (int kz = -kr; kz <= kr; ++kz) {
+            const int sz = kz * a.nc;
+            const int z0 = max(r.lo[2], cz0 + sz), z1 = min(r.hi[2], cz0 + sz + T);
+            if (z0 >= z1) continue;
+            for (int ky = -kr; ky <= kr; ++ky) {
+                const int sy = ky * a.nc;
+                const int y0 = max(r.lo[1], cy0 + sy), y1 = min(r.hi[1], cy0 + sy + T);
+                if (y0 >= y1) continue;
+                for (int kx = -kr; kx <= kr; ++kx) {
+                    const int sx = kx * a.nc;
+                    const int x0 = max(r.lo[0], cx0 + sx), x1 = min(r.hi[0], cx0 + sx + T);
+                    if (x0 >= x1) continue;
+                    // x weights of this row segment, reused for every (z, y).
+                    for (int ix = x0; ix < x1; ++ix) {
+                        const float s1 = (static_cast<float>(ix) * dxc - r.x[0]) * hinv;
+                        wxs[tid][ix - x0] = ker_integ(s1, s1 + ds);
+                    }
+                    for (int iz = z0; iz < z1; ++iz) {
+                        const float s1z = (static_cast<float>(iz) * dxc - r.x[2]) * hinv;
+                        const float kzw = ker_integ(s1z, s1z + ds);
+                        for (int iy = y0; iy < y1; ++iy) {
+                            const float s1y = (static_cast<float>(iy) * dxc - r.x[1]) * hinv;
+                            const float kyw = ker_integ(s1y, s1y + ds);
+                            const int rowc = ((iz - sz - cz0) * T + (iy - sy - cy0)) * T - cx0 - sx;
+                            for (int ix = x0; ix < x1; ++ix) {
+                                const float kxw = wxs[tid][ix - x0];
+                                float ker3 = kxw * kyw;
+                                ker3 = ker3 * kzw;
+                                const int c = rowc + ix;
+                                if (igm) {
+                                    atomicAdd(&acc[0][c], static_cast<double>(ker3));
+                                    atomicAdd(&acc[2][c], static_cast<double>(ker3 * dens));
+                                    atomicAdd(&acc[3][c], static_cast<double>(ker3 * at.x));
+                                    atomicAdd(&acc[4][c], static_cast<double>(ker3 * at.y));
+                                    atomicAdd(&acc[5][c], static_cast<double>(ker3 * at.z));
+                                } else {
+                                    atomicAdd(&acc[1][c], static_cast<double>(ker3));
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int64_t nc = a.nc;
+    const int64_t nmesh = nc * nc * nc;
+    for (int c = tid; c < ncell; c += kDepThreads) {
+        const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
+        const int gx = cx0 + lx, gy = cy0 + ly, gz = cz0 + lz;
+        if (gx >= a.nc || gy >= a.nc || gz >= a.nc) continue;
+        const int64_t idx = (static_cast<int64_t>(gz) * nc + gy) * nc + gx;
+        float vals[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) vals[f] = static_cast<float>(acc[f][c]);
+        if (accumulate) {
+            mesh[idx] += vals[0];
+            mesh[nmesh + idx] += vals[1];
+            mesh[2 * nmesh + idx] += vals[2];
+            mesh[3 * nmesh + 3 * idx + 0] += vals[3];
+            mesh[3 * nmesh + 3 * idx + 1] += vals[4];
+            mesh[3 * nmesh + 3 * idx + 2] += vals[5];
+        } else {
+            mesh[idx] = vals[0];
+            mesh[nmesh + idx] = vals[1];
+            mesh[2 * nmesh + idx] = vals[2];
+            mesh[3 * nmesh + 3 * idx + 0] = vals[3];
+            mesh[3 * nmesh + 3 * idx + 1] = vals[4];
+            mesh[3 * nmesh + 3 * idx + 2] = vals[5];
+        }
+    }
+}
+
+template <typename T>
+__global__ void fill_zero_kernel(T* __restrict__ p, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 1024;
+
+// Stage 1: per-workgroup FP64 partial sums of n, halo, n^2, c over a
+// grid-stride range; stage 2 adds the partials in a fixed order.
+__global__ __launch_bounds__(kRedThreads) void mesh_reduce_stage1(const float* __restrict__ mesh,
+                                                                  int64_t nmesh,
+                                                                  double* __restrict__ partial)
+{
+    __shared__ double sh[4][kRedThreads];
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int64_t j = static_cast<int64_t>(blockIdx.x) * kRedThreads + threadIdx.x; j < nmesh;
+         j += static_cast<int64_t>(gridDim.x) * kRedThreads) {
+        const double nv = mesh[j];
+        s0 += nv;
+        s1 += static_cast<double>(mesh[nmesh + j]);
+        s2 += nv * nv;
+        s3 += static_cast<double>(mesh[2 * nmesh + j]);
+    }
+    sh[0][threadIdx.x] = s0;
+    sh[1][threadIdx.x] = s1;
+    sh[2][threadIdx.x] = s2;
+    sh[3][threadIdx.x] = s3;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int f = 0; f < 4; ++f) sh[f][threadIdx.x] += sh[f][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partial[threadIdx.x * kRedBlocks + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void mesh_reduce_stage2(const double* __restrict__ partial,
+                                                                  double* __restrict__ out)
+{
+    __shared__ double sh[kRedThreads];
+    for (int f = 0; f < 4; ++f) {
+        double s = 0;
+        for (int b = threadIdx.x; b < kRedBlocks; b += kRedThreads) s += partial[f * kRedBlocks + b];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[f] = sh[0];
+        __syncthreads();
+    }
+}
+
+// coarse_mesh.cpp:158-171 in FP32, operation for operation.
+__global__ void normalize_kernel(float* __restrict__ mesh, int64_t nmesh, float vol)
+{
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nmesh) return;
+    float* n = mesh;
+    float* halo = mesh + nmesh;
+    float* c = halo + nmesh;
+    float* v = c + nmesh;
+    const float nj = n[j];
+    if (nj > 0.0f) {
+        c[j] = c[j] * (vol / (nj * nj));
+        v[3 * j] = v[3 * j] / nj;
+        v[3 * j + 1] = v[3 * j + 1] / nj;
+        v[3 * j + 2] = v[3 * j + 2] / nj;
+    } else {
+        c[j] = 0.0f;
+        v[3 * j] = 0.0f;
+        v[3 * j + 1] = 0.0f;
+        v[3 * j + 2] = 0.0f;
+    }
+    n[j] = nj * 8.0f;
+    halo[j] = halo[j] * 8.0f;
+}
+
+}  // namespace
+
+hipError_t launch_deposit_count(const DepositArgs& a, uint32_t* tile_cnt, hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    const int th = 256;
+    deposit_count_kernel<<<static_cast<unsigned>((a.n + th - 1) / th), th, 0, s>>>(a, tile_cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start, uint32_t* tile_cur,
+                               uint32_t* list, hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    const int th = 256;
+    deposit_fill_kernel<<<static_cast<unsigned>((a.n + th - 1) / th), th, 0, s>>>(a, tile_start, tile_cur, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start, const uint32_t* tile_cnt,
+                                const uint32_t* list, float* mesh, int accumulate, hipStream_t s)
+{
+    if (a.tile < 1 || a.tile > kTileMax) return hipErrorInvalidValue;
+    const int64_t ntiles = static_cast<int64_t>(a.ntx) * a.ntx * a.ntx;
+    deposit_tile_kernel<<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(a, tile_start, tile_cnt, list,
+                                                                              mesh, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    fill_zero_kernel<uint32_t><<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero_f32(float* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    fill_zero_kernel<float><<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero_f64(double* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    fill_zero_kernel<double><<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_mesh_reduce(const float* mesh, int nc, double* partial, double* out, hipStream_t s)
+{
+    const int64_t nmesh = static_cast<int64_t>(nc) * nc * nc;
+    mesh_reduce_stage1<<<kRedBlocks, kRedThreads, 0, s>>>(mesh, nmesh, partial);
+    ASM_HIP_CHECK(hipGetLastError());
+    mesh_reduce_stage2<<<1, kRedThreads, 0, s>>>(partial, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize(float* mesh, int nc, float vol, hipStream_t s)
+{
+    const int64_t nmesh = static_cast<int64_t>(nc) * nc * nc;
+    normalize_kernel<<<static_cast<unsigned>((nmesh + 255) / 256), 256, 0, s>>>(mesh, nmesh, vol);
+    return hipGetLastError();
+}
+
+}  // namespace asm_hip

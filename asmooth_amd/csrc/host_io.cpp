@@ -1,0 +1,245 @@
+// Host-side steps either side of the device path: the buffered particle set
+// of a node, the cubep3m xv reader/writer and the mesh file writer.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/asmooth.h"
+
+namespace {
+
+// Region of one axis: 0 below a, 2 above b, 1 between (exchange_buffer.cpp:6-11).
+inline int axis_region(float u, float a, float b)
+{
+    return (u >= a) + (u > b);
+}
+
+inline int region_index(const float* u, float a, float b)
+{
+    return axis_region(u[0], a, b) + 3 * axis_region(u[1], a, b) + 9 * axis_region(u[2], a, b);
+}
+
+// Direction j (0..26, 13 = none) may carry a particle of region i when, on
+// every axis, it either does not move or moves toward the face the particle
+// is near (exchange_buffer.cpp:123-126).
+inline bool direction_carries(int i, int j)
+{
+    if (i == 13 || j == 13) return false;
+    const int bn[3] = {i % 3, (i / 3) % 3, i / 9};
+    const int dn[3] = {j % 3, (j / 3) % 3, j / 9};
+    for (int a = 0; a < 3; ++a)
+        if (!(dn[a] == 1 || dn[a] == bn[a])) return false;
+    return true;
+}
+
+inline asm_particle make_rec(const float* u, const float* v)
+{
+    asm_particle p;
+    for (int a = 0; a < 3; ++a) {
+        p.x[a] = u[a];
+        p.v[a] = v[a];
+    }
+    p.dens = 0.0f;
+    p.rk = 0.0f;
+    p.igrp = 0;
+    return p;
+}
+
+// remove_extra_particles (clumping_tree_main.cpp:550-576): keep records
+// strictly inside (-bf L, L + bf L)^3, order preserved.
+int64_t keep_inside(std::vector<asm_particle>& all, int64_t nlocal_in, float boxsize, float bf,
+                    asm_particle* out, int64_t cap, int64_t* np_local)
+{
+    const float dx = bf * boxsize;
+    const float left = -dx;
+    const float right = boxsize + dx;
+    int64_t nkeep = 0;
+    for (const asm_particle& p : all) {
+        if (left < p.x[0] && p.x[0] < right && left < p.x[1] && p.x[1] < right && left < p.x[2] &&
+            p.x[2] < right) {
+            if (nkeep >= cap) return ASM_ERR_CAPACITY;
+            out[nkeep++] = p;
+        }
+    }
+    if (np_local) *np_local = nlocal_in - (static_cast<int64_t>(all.size()) - nkeep);
+    return nkeep;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t asm_build_buffered_set(const float* xv, int64_t np, float boxsize, float bf,
+                               asm_particle* out, int64_t cap, int64_t* np_local)
+{
+    if (!xv || !out || np < 0 || !(boxsize > 0.0f) || !(bf > 0.0f)) return ASM_ERR_ARG;
+    const float a = bf * boxsize;                 // exchange_buffer.cpp:23
+    const float b = (1.0f - bf) * boxsize;        // exchange_buffer.cpp:24
+
+    // Buffer candidates in the order the reference's copy loop leaves them:
+    // written from the end of the allocation downwards, i.e. reverse file
+    // order (exchange_buffer.cpp:36-47).
+    std::vector<int64_t> q;
+    int64_t nbuf[27] = {0};
+    for (int64_t i = np - 1; i >= 0; --i) {
+        const int r = region_index(xv + 6 * i, a, b);
+        if (r != 13) {
+            q.push_back(i);
+            nbuf[r]++;
+        }
+    }
+    // The reference's in-place cycle sort by region (exchange_buffer.cpp:60-93),
+    // kept so that equal distances resolve to the same indices.
+    int64_t next[27], begin[27], end[27];
+    next[0] = begin[0] = 0;
+    end[0] = nbuf[0];
+    for (int i = 1; i < 27; ++i) {
+        next[i] = next[i - 1] + nbuf[i - 1];
+        begin[i] = next[i];
+        end[i] = next[i] + nbuf[i];
+    }
+    for (int i = 0; i < 27; ++i) {
+        for (int64_t j = next[i]; j < end[i]; ++j) {
+            int r = region_index(xv + 6 * q[j], a, b);
+            while (r != i) {
+                const int64_t k = next[r]++;
+                std::swap(q[k], q[j]);
+                r = region_index(xv + 6 * q[j], a, b);
+            }
+        }
+    }
+
+    std::vector<asm_particle> all;
+    all.reserve(static_cast<size_t>(np + q.size() * 2));
+    for (int64_t i = 0; i < np; ++i) all.push_back(make_rec(xv + 6 * i, xv + 6 * i + 3));
+    // Images per (source region i, direction j) (exchange_buffer.cpp:110-151),
+    // shifted by -(dn-1) L (:138-144).
+    for (int i = 0; i < 27; ++i) {
+        for (int j = 0; j < 27; ++j) {
+            if (!direction_carries(i, j)) continue;
+            const float s[3] = {static_cast<float>(j % 3 - 1) * boxsize,
+                                static_cast<float>((j / 3) % 3 - 1) * boxsize,
+                                static_cast<float>(j / 9 - 1) * boxsize};
+            for (int64_t k = begin[i]; k < begin[i] + nbuf[i]; ++k) {
+                const float* p = xv + 6 * q[k];
+                const float u[3] = {p[0] - s[0], p[1] - s[1], p[2] - s[2]};
+                all.push_back(make_rec(u, p + 3));
+            }
+        }
+    }
+    return keep_inside(all, np, boxsize, bf, out, cap, np_local);
+}
+
+int64_t asm_build_node_share(const float* xv, int64_t np, float box_total, int nn, const int coord[3],
+                             float bf, asm_particle* out, int64_t cap, int64_t* np_local)
+{
+    if (!xv || !out || np < 0 || nn < 1 || !(box_total > 0.0f) || !(bf > 0.0f)) return ASM_ERR_ARG;
+    for (int a = 0; a < 3; ++a)
+        if (coord[a] < 0 || coord[a] >= nn) return ASM_ERR_ARG;
+    const float L = box_total / static_cast<float>(nn);
+    const float a = bf * L;
+    const float b = (1.0f - bf) * L;
+    std::vector<asm_particle> local;
+    std::vector<std::vector<asm_particle>> images(27 * 27);
+    for (int64_t i = 0; i < np; ++i) {
+        const float* p = xv + 6 * i;
+        int home[3];
+        float u[3];
+        for (int d = 0; d < 3; ++d) {
+            int h = static_cast<int>(p[d] / L);
+            if (h < 0) h = 0;
+            if (h >= nn) h = nn - 1;
+            home[d] = h;
+            u[d] = p[d] - static_cast<float>(h) * L;
+        }
+        if (home[0] == coord[0] && home[1] == coord[1] && home[2] == coord[2])
+            local.push_back(make_rec(u, p + 3));
+        const int r = region_index(u, a, b);
+        if (r == 13) continue;
+        for (int j = 0; j < 27; ++j) {
+            if (!direction_carries(r, j)) continue;
+            const int dn[3] = {j % 3 - 1, (j / 3) % 3 - 1, j / 9 - 1};
+            bool to_me = true;
+            for (int d = 0; d < 3; ++d)
+                if (((home[d] + dn[d]) % nn + nn) % nn != coord[d]) to_me = false;
+            if (!to_me) continue;
+            const float w[3] = {u[0] - static_cast<float>(dn[0]) * L, u[1] - static_cast<float>(dn[1]) * L,
+                                u[2] - static_cast<float>(dn[2]) * L};
+            images[r * 27 + j].push_back(make_rec(w, p + 3));
+        }
+    }
+    std::vector<asm_particle> all = local;
+    for (auto& v : images) all.insert(all.end(), v.begin(), v.end());
+    return keep_inside(all, static_cast<int64_t>(local.size()), L, bf, out, cap, np_local);
+}
+
+int asm_read_xv(const char* path, float** xv, int64_t* np, float header[11])
+{
+    if (!path || !xv || !np) return ASM_ERR_ARG;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return ASM_ERR_IO;
+    int32_t n = 0;
+    float hdr[11];
+    if (std::fread(&n, sizeof(int32_t), 1, fp) != 1 || std::fread(hdr, sizeof(float), 11, fp) != 11 || n < 0) {
+        std::fclose(fp);
+        return ASM_ERR_IO;
+    }
+    float* buf = static_cast<float*>(std::malloc(sizeof(float) * 6 * static_cast<size_t>(n > 0 ? n : 1)));
+    if (!buf) {
+        std::fclose(fp);
+        return ASM_ERR_CAPACITY;
+    }
+    if (std::fread(buf, sizeof(float), 6 * static_cast<size_t>(n), fp) != 6 * static_cast<size_t>(n)) {
+        std::free(buf);
+        std::fclose(fp);
+        return ASM_ERR_IO;
+    }
+    std::fclose(fp);
+    *xv = buf;
+    *np = n;
+    if (header) std::memcpy(header, hdr, sizeof(hdr));
+    return ASM_OK;
+}
+
+int asm_write_xv(const char* path, const float* xv, int64_t np, const float header[11])
+{
+    if (!path || !xv || np < 0 || np > 0x7fffffff) return ASM_ERR_ARG;
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return ASM_ERR_IO;
+    const int32_t n = static_cast<int32_t>(np);
+    float hdr[11] = {0};
+    if (header) std::memcpy(hdr, header, sizeof(hdr));
+    bool ok = std::fwrite(&n, sizeof(int32_t), 1, fp) == 1 && std::fwrite(hdr, sizeof(float), 11, fp) == 11 &&
+              std::fwrite(xv, sizeof(float), 6 * static_cast<size_t>(np), fp) == 6 * static_cast<size_t>(np);
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? ASM_OK : ASM_ERR_IO;
+}
+
+void asm_free(void* p)
+{
+    std::free(p);
+}
+
+int asm_write_mesh_separate(const char* base, int inode, const float* mesh, int nc)
+{
+    if (!base || !mesh || nc <= 0) return ASM_ERR_ARG;
+    const size_t nmesh = static_cast<size_t>(nc) * nc * nc;
+    const char* tags[4] = {"n", "h", "c", "vel"};
+    const float* parts[4] = {mesh, mesh + nmesh, mesh + 2 * nmesh, mesh + 3 * nmesh};
+    const size_t counts[4] = {nmesh, nmesh, nmesh, 3 * nmesh};
+    for (int f = 0; f < 4; ++f) {
+        std::vector<char> name(std::strlen(base) + 32);
+        std::snprintf(name.data(), name.size(), "%s%s%d.dat", base, tags[f], inode);
+        FILE* fp = std::fopen(name.data(), "wb");
+        if (!fp) return ASM_ERR_IO;
+        const int32_t n = nc;
+        bool ok = std::fwrite(&n, sizeof(int32_t), 1, fp) == 1 &&
+                  std::fwrite(parts[f], sizeof(float), counts[f], fp) == counts[f];
+        ok = (std::fclose(fp) == 0) && ok;
+        if (!ok) return ASM_ERR_IO;
+    }
+    return ASM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Host-side launchers of the gfx950 kernels (one translation unit per stage).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace asm_hip {
+
+// ---- sort.hip --------------------------------------------------------------
+hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s);
+int64_t radix_hist_entries(int64_t n);
+hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
+                             int64_t n, int bits, uint32_t* hist, hipStream_t s, bool* result_in_tmp);
+
+// ---- tree.hip --------------------------------------------------------------
+// The caller's particle record, byte-compatible with the reference's
+// `Particle` (basic_types.h:10-15): x[3], v[3], dens, rk, igrp.
+struct ParticleRec {
+    float x[3];
+    float v[3];
+    float dens;
+    float rk;
+    int igrp;
+};
+static_assert(sizeof(ParticleRec) == 36, "Particle record must be 36 bytes");
+
+hipError_t launch_morton_keys(const ParticleRec* p, int64_t n, float left, float right,
+                              uint32_t* keys, uint32_t* vals, hipStream_t s);
+// spos = (x, y, z, original index bits); sattr = (vx, vy, vz, igrp bits);
+// inv_perm[orig] = sorted position.
+hipError_t launch_gather_sorted(const ParticleRec* p, const uint32_t* perm, int64_t n,
+                                float4* spos, float4* sattr, int32_t* inv_perm, hipStream_t s);
+hipError_t launch_build_tree(const float4* spos, int64_t n, int leaf_size, int64_t nleaf,
+                             int64_t nleaf_pad, Box* nodes, hipStream_t s);
+
+// ---- knn.hip ---------------------------------------------------------------
+struct KnnStats {
+    unsigned long long node_tests;
+    unsigned long long leaf_scans;
+    unsigned long long dist_evals;
+    unsigned long long inserts;
+};
+hipError_t launch_knn_dens(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad,
+                           const int32_t* inv_perm, float* rk_sorted, double* dens64,
+                           int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s);
+// dens (sorted order, float) from the FP64 sums.
+hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s);
+
+// ---- deposit.hip -------------------------------------------------------------
+struct DepositArgs {
+    const float4* spos;
+    const float4* sattr;
+    const float* rk;
+    const float* dens;
+    int64_t n;
+    float boxsize;
+    int nc;
+    int sign;
+    int tile;        // tile side in cells
+    int ntx;         // tiles per axis
+    int periodic;    // 0: node mesh (coarse_mesh.cpp), 1: periodic variant (global_mesh.cpp)
+    int coord[3];    // node coordinate for the periodic variant
+};
+hipError_t launch_deposit_count(const DepositArgs& a, uint32_t* tile_cnt, hipStream_t s);
+hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start, uint32_t* tile_cur,
+                               uint32_t* list, hipStream_t s);
+hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start, const uint32_t* tile_cnt,
+                                const uint32_t* list, float* mesh, int accumulate, hipStream_t s);
+hipError_t launch_fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s);
+hipError_t launch_fill_zero_f32(float* p, int64_t n, hipStream_t s);
+hipError_t launch_fill_zero_f64(double* p, int64_t n, hipStream_t s);
+// FP64 reductions of a mesh: out[0] = sum n, out[1] = sum halo, out[2] = sum n^2,
+// out[3] = sum c (raw).  partial must hold 4 * 1024 doubles.
+hipError_t launch_mesh_reduce(const float* mesh, int nc, double* partial, double* out, hipStream_t s);
+hipError_t launch_normalize(float* mesh, int nc, float vol, hipStream_t s);
+
+// ---- halo.hip ----------------------------------------------------------------
+struct HaloRec {
+    float x[3];
+    float r;
+    float mass;
+};
+hipError_t launch_mark_halo(const float4* spos, float4* sattr, int64_t n, const Box* nodes,
+                            int64_t nleaf_pad, const HaloRec* halos, int64_t nh, float left,
+                            float right, unsigned long long* count, hipStream_t s);
+
+}  // namespace asm_hip

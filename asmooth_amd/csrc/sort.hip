@@ -1,0 +1,233 @@
+// LSD radix sort of 32-bit keys with 32-bit payloads, and a single-workgroup
+// exclusive scan.  Used to put particles in Morton order (the tree build) and
+// to turn per-tile counts into list offsets (mesh deposition).
+//
+// Each pass handles 4 key bits.  A workgroup owns a tile of 4096 keys; thread
+// t owns 16 consecutive keys, so the in-tile order is (thread, item) and the
+// per-(digit, thread) counts scanned in digit-major order give every key a
+// stable destination.  Per-tile digit totals are scanned across tiles in
+// digit-major order, which makes each pass stable across the whole array.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace asm_hip {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr int kRadix = 16;
+constexpr int kCntStride = kSortThreads + 1;          // padded digit rows
+
+__global__ __launch_bounds__(kSortThreads) void rs_hist(const uint32_t* __restrict__ keys,
+                                                        int64_t n, int shift,
+                                                        uint32_t* __restrict__ hist, int nb)
+{
+    __shared__ uint32_t h[kSortThreads / kWave][kRadix];
+    const int t = threadIdx.x;
+    const int w = t / kWave;
+    if (t < (kSortThreads / kWave) * kRadix) (&h[0][0])[t] = 0u;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+#pragma unroll 4
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = base + static_cast<int64_t>(i) * kSortThreads + t;
+        if (idx < n) atomicAdd(&h[w][(keys[idx] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    if (t < kRadix) {
+        uint32_t s = 0;
+        for (int ww = 0; ww < kSortThreads / kWave; ++ww) s += h[ww][t];
+        hist[static_cast<int64_t>(t) * nb + blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of one value per thread across a 256-thread workgroup.
+__device__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* wsum /* [4] */, uint32_t* total)
+{
+    const int lane = threadIdx.x % kWave;
+    const int w = threadIdx.x / kWave;
+    uint32_t inc = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int ww = 0; ww < kSortThreads / kWave; ++ww) {
+        if (ww < w) pre += wsum[ww];
+        tot += wsum[ww];
+    }
+    if (total) *total = tot;
+    return pre + inc - v;
+}
+
+__global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout,
+                                                           int64_t n, int shift,
+                                                           const uint32_t* __restrict__ hist, int nb)
+{
+    __shared__ uint32_t cnt[kRadix * kCntStride];
+    __shared__ uint32_t wsum[kSortThreads / kWave];
+    __shared__ uint32_t gbase[kRadix];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < kRadix; ++d) cnt[d * kCntStride + t] = 0u;
+    if (t < kRadix) gbase[t] = hist[static_cast<int64_t>(t) * nb + blockIdx.x];
+
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile + static_cast<int64_t>(t) * kSortItems;
+    uint32_t k[kSortItems], v[kSortItems];
+    int nvalid = 0;
+    if (base + kSortItems <= n) {
+        const uint4* kp = reinterpret_cast<const uint4*>(kin + base);
+        const uint4* vp = reinterpret_cast<const uint4*>(vin + base);
+#pragma unroll
+        for (int q = 0; q < kSortItems / 4; ++q) {
+            const uint4 a = kp[q];
+            const uint4 b = vp[q];
+            k[4 * q + 0] = a.x; k[4 * q + 1] = a.y; k[4 * q + 2] = a.z; k[4 * q + 3] = a.w;
+            v[4 * q + 0] = b.x; v[4 * q + 1] = b.y; v[4 * q + 2] = b.z; v[4 * q + 3] = b.w;
+        }
+        nvalid = kSortItems;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const int64_t idx = base + j;
+            const bool ok = idx < n;
+            k[j] = ok ? kin[idx] : 0u;
+            v[j] = ok ? vin[idx] : 0u;
+        }
+        const int64_t rem = n - base;
+        nvalid = rem <= 0 ? 0 : static_cast<int>(rem < kSortItems ? rem : kSortItems);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j)
+        if (j < nvalid) cnt[((k[j] >> shift) & (kRadix - 1)) * kCntStride + t] += 1u;
+    __syncthreads();
+
+    // Digit-major exclusive scan of the 16 x 256 counts: thread t scans the
+    // 16 consecutive entries e = 16t .. 16t+15 (digit t/16, threads 16(t%16)+i).
+    const int d0 = t / kRadix;
+    const int t0 = (t % kRadix) * kRadix;
+    uint32_t s[kRadix];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kRadix; ++i) {
+        s[i] = cnt[d0 * kCntStride + t0 + i];
+        sum += s[i];
+    }
+    uint32_t run = block_exclusive_scan_256(sum, wsum, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRadix; ++i) {
+        cnt[d0 * kCntStride + t0 + i] = run;
+        run += s[i];
+    }
+    __syncthreads();
+    // cnt now holds the in-tile start of each (digit, thread) run.  The tile's
+    // digit-d keys start at gbase[d] - (in-tile start of digit d, thread 0).
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        if (j >= nvalid) break;
+        const uint32_t d = (k[j] >> shift) & (kRadix - 1);
+        const uint32_t local = cnt[d * kCntStride + t] - cnt[d * kCntStride];
+        cnt[d * kCntStride + t] += 1u;
+        const int64_t pos = static_cast<int64_t>(gbase[d]) + local;
+        kout[pos] = k[j];
+        vout[pos] = v[j];
+    }
+}
+
+constexpr int kScanThreads = 1024;
+
+// In-place exclusive scan of m uint32 values by one workgroup; writes the
+// grand total to *total (if non-null).
+__global__ __launch_bounds__(kScanThreads) void scan_exclusive_1wg(uint32_t* __restrict__ a, int64_t m,
+                                                                   uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t wsum[kScanThreads / kWave];
+    __shared__ uint32_t carry_s;
+    const int t = threadIdx.x;
+    const int lane = t % kWave;
+    const int w = t / kWave;
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    constexpr int kPer = 4;
+    for (int64_t chunk = 0; chunk < m; chunk += static_cast<int64_t>(kScanThreads) * kPer) {
+        const int64_t b = chunk + static_cast<int64_t>(t) * kPer;
+        uint32_t x[kPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            x[i] = (b + i < m) ? a[b + i] : 0u;
+            sum += x[i];
+        }
+        uint32_t inc = sum;
+        for (int o = 1; o < kWave; o <<= 1) {
+            uint32_t y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == kWave - 1) wsum[w] = inc;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int ww = 0; ww < kScanThreads / kWave; ++ww) {
+            if (ww < w) pre += wsum[ww];
+            tot += wsum[ww];
+        }
+        const uint32_t carry = carry_s;
+        uint32_t run = carry + pre + inc - sum;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            if (b + i < m) a[b + i] = run;
+            run += x[i];
+        }
+        __syncthreads();
+        if (t == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (t == 0 && total) *total = carry_s;
+}
+
+}  // namespace
+
+hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    scan_exclusive_1wg<<<1, kScanThreads, 0, s>>>(a, m, total);
+    return hipGetLastError();
+}
+
+int64_t radix_hist_entries(int64_t n)
+{
+    const int64_t nb = (n + kSortTile - 1) / kSortTile;
+    return nb * kRadix;
+}
+
+hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
+                             int64_t n, int bits, uint32_t* hist, hipStream_t s, bool* result_in_tmp)
+{
+    *result_in_tmp = false;
+    if (n <= 1) return hipSuccess;
+    const int nb = static_cast<int>((n + kSortTile - 1) / kSortTile);
+    uint32_t* kin = keys;
+    uint32_t* vin = vals;
+    uint32_t* kout = keys_tmp;
+    uint32_t* vout = vals_tmp;
+    for (int shift = 0; shift < bits; shift += 4) {
+        rs_hist<<<nb, kSortThreads, 0, s>>>(kin, n, shift, hist, nb);
+        ASM_HIP_CHECK(hipGetLastError());
+        ASM_HIP_CHECK(launch_scan_exclusive(hist, static_cast<int64_t>(nb) * kRadix, nullptr, s));
+        rs_scatter<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
+        ASM_HIP_CHECK(hipGetLastError());
+        uint32_t* tk = kin; kin = kout; kout = tk;
+        uint32_t* tv = vin; vin = vout; vout = tv;
+    }
+    *result_in_tmp = (kin == keys_tmp);
+    return hipSuccess;
+}
+
+}  // namespace asm_hip

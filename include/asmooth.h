@@ -1,0 +1,175 @@
+/*
+ * asmooth_amd - C ABI of the MI355X (gfx950) adaptive-smoothing path.
+ *
+ * The reference (junkoda/asmooth) has no plugin or FFI layer: its hot path is a
+ * fixed sequence of C++ calls made by for_each_redshift
+ * (clumping_tree_main.cpp:390-533).  Each entry point below replaces one of
+ * those calls, cited next to it; the caller (a C++ driver, or Python through
+ * ctypes) makes them in the same order:
+ *
+ *   asm_set_particles         Particles after exchange_buffer + remove_extra
+ *   asm_tree_construct        KDTree::construct
+ *   asm_neighbor_search       the neighbor_search loop
+ *   asm_mark_halo_particles   the mark_halo_particles loop (HALO_EXCISE)
+ *   asm_assign_on_mesh        clear_mesh + assign_on_mesh, per nc
+ *   asm_mesh_total            mesh_total (n and halo)
+ *   asm_normalize_mesh        the normalisation inside write_mesh_separate
+ *   asm_get_mesh / asm_write_mesh_separate   the mesh files
+ *
+ * Conventions: every function returns ASM_OK (0) or a negative status and
+ * never aborts; asm_last_error() describes the last failure on a context.
+ * The caller owns host buffers; the context owns device buffers (no globals),
+ * reuses them across calls, and runs everything on one HIP stream of its own.
+ * One context per host thread.  Arrays of particles are in the caller's order
+ * (the device copy is Morton-sorted internally and never exposed).
+ */
+#ifndef ASMOOTH_AMD_H
+#define ASMOOTH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Particle record, byte-identical to the reference's Particle
+ * (basic_types.h:10-15): position, velocity, density, smoothing length, group. */
+typedef struct asm_particle {
+    float x[3];
+    float v[3];
+    float dens;
+    float rk;
+    int32_t igrp;
+} asm_particle;
+
+/* Halo record, byte-identical to the reference's Halo (basic_types.h:17-20). */
+typedef struct asm_halo {
+    float x[3];
+    float r;
+    float mass;
+} asm_halo;
+
+typedef struct asm_context asm_context;
+
+enum {
+    ASM_OK = 0,
+    ASM_ERR_ARG = -1,      /* invalid argument */
+    ASM_ERR_HIP = -2,      /* HIP runtime failure (message holds the HIP error) */
+    ASM_ERR_STATE = -3,    /* call out of order (e.g. search before construct) */
+    ASM_ERR_CAPACITY = -4, /* size beyond what the library supports */
+    ASM_ERR_IO = -5        /* file could not be read or written */
+};
+
+/* Options for asm_set_option. */
+enum {
+    ASM_OPT_KEEP_NEIGHBORS = 1, /* 1: asm_neighbor_search keeps the 33-slot lists */
+    ASM_OPT_PROFILE = 2,        /* 1: time every kernel with HIP events */
+    ASM_OPT_LEAF_SIZE = 3,      /* particles per tree leaf, 8..64 (default 32) */
+    ASM_OPT_TILE = 4            /* deposition tile side in cells, 1..8 (default 8) */
+};
+
+const char* asm_version(void);
+
+/* Context on HIP device `device` (ordinal). */
+int asm_create(asm_context** ctx, int device);
+void asm_destroy(asm_context* ctx);
+const char* asm_last_error(const asm_context* ctx);
+int asm_set_option(asm_context* ctx, int option, int value);
+int asm_synchronize(asm_context* ctx);
+
+/* The buffered particle set of one node (Particles, basic_types.h:26-37, as
+ * it stands after exchange_buffer and remove_extra_particles,
+ * clumping_tree_main.cpp:376-385): np_with_buffers records whose first
+ * np_local are the node's own.  Copied to the device; dens is reset to 0 on
+ * the device (the reference relies on the reader, file_reader2.cpp:80). */
+int asm_set_particles(asm_context* ctx, const asm_particle* particles, int64_t np_with_buffers,
+                      int64_t np_local, float boxsize);
+/* Same, from records already in device memory on the context's device. */
+int asm_set_particles_device(asm_context* ctx, const asm_particle* d_particles,
+                             int64_t np_with_buffers, int64_t np_local, float boxsize);
+
+/* KDTree::construct (kdtree_balanced.h:18; called at clumping_tree_main.cpp:392-394
+ * with left = -bf*L, right = (1+bf)*L).  Builds the device tree; *nnode
+ * receives the number of tree nodes. */
+int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode);
+
+/* The neighbour loop (clumping_tree_main.cpp:418-430 calling neighbor_search,
+ * nbr_search.h:9): rk = distance to the k-th nearest other particle and the
+ * tent-kernel density scatter to the k nearest (self included).  k must be 32
+ * (KNeighbors::k = 33 slots, k_neighbors.h:47).  Ties are broken by index. */
+int asm_neighbor_search(asm_context* ctx, int k);
+
+/* mark_halo_particles over a halo catalogue (nbr_search.h:11; loop
+ * clumping_tree_main.cpp:465-485): igrp = -1 for particles with d2 < r*r of a
+ * halo whose centre lies strictly inside (left, right)^3.  *np_halo = number
+ * of (halo, particle) marks, *m_halo = sum of mass/8 over those halos. */
+int asm_mark_halo_particles(asm_context* ctx, const asm_halo* halos, int64_t n_halo, float left,
+                            float right, int64_t* np_halo, double* m_halo);
+
+/* clear_mesh + assign_on_mesh (coarse_mesh.h:4,6; clumping_tree_main.cpp:525-526)
+ * into the context's device mesh for `nc` (6*nc^3 floats: n | halo | c | v
+ * with v xyz-interleaved).  sign as in the reference (-1 for the SO pass). */
+int asm_assign_on_mesh(asm_context* ctx, int nc, int sign);
+/* Every nc of a -nc list in one call (clumping_tree_main.cpp:521-533). */
+int asm_assign_on_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign);
+/* Periodic node-mesh variant (assign_on_global_mesh, global_mesh.cpp:25-123):
+ * local particles only, node offset coord*L, cell indices wrapped. */
+int asm_assign_on_global_mesh(asm_context* ctx, int nc, int sign, const int coord[3]);
+
+/* mesh_total (coarse_mesh.h:5) of n and of halo, in FP64 (check_total_mass,
+ * clumping_tree_main.cpp:578-600). */
+int asm_mesh_total(asm_context* ctx, int nc, double totals[2]);
+/* FP64 reductions of the raw mesh: {sum n, sum halo, sum n^2, sum c}. */
+int asm_mesh_stats(asm_context* ctx, int nc, double out[4]);
+/* The normalisation of write_mesh_separate (coarse_mesh.cpp:150-171), in place. */
+int asm_normalize_mesh(asm_context* ctx, int nc);
+/* Copy the mesh for nc (6*nc^3 floats) to host memory. */
+int asm_get_mesh(asm_context* ctx, int nc, float* mesh);
+/* Device address of the mesh for nc (valid until the context is destroyed). */
+const float* asm_mesh_device_ptr(asm_context* ctx, int nc);
+
+/* Results per particle in the caller's order: rk, dens and igrp are written,
+ * x and v are copied back unchanged. */
+int asm_get_particles(asm_context* ctx, asm_particle* particles);
+int asm_get_rk_dens(asm_context* ctx, float* rk, float* dens, int32_t* igrp);
+/* Neighbour lists (np_with_buffers x 33 original indices, ascending (d2, index))
+ * kept by the last search when ASM_OPT_KEEP_NEIGHBORS is set. */
+int asm_get_neighbors(asm_context* ctx, int32_t* nbr);
+
+/* Host steps either side of the path. */
+/* Buffered set of a single periodic rank: the 26 periodic images of
+ * exchange_buffer (exchange_buffer.cpp:13-167, one rank) followed by
+ * remove_extra_particles (clumping_tree_main.cpp:550-576).  xv: np x 6 floats.
+ * Returns the buffered count, or a negative status. */
+int64_t asm_build_buffered_set(const float* xv, int64_t np, float boxsize, float bf,
+                               asm_particle* out, int64_t cap, int64_t* np_local);
+/* Node share of a periodic box cut into nn^3 node cubes: node `coord` gets its
+ * own particles and every periodic image within bf*Lnode of its cube, in node
+ * coordinates (what the reference holds after the exchange on that rank). */
+int64_t asm_build_node_share(const float* xv, int64_t np, float box_total, int nn,
+                             const int coord[3], float bf, asm_particle* out, int64_t cap,
+                             int64_t* np_local);
+/* cubep3m xv reader (file_reader2.cpp:24-86): header = int32 np + 11 float32,
+ * then np x {x,y,z,vx,vy,vz}.  *xv is malloc'ed; free with asm_free. */
+int asm_read_xv(const char* path, float** xv, int64_t* np, float header[11]);
+int asm_write_xv(const char* path, const float* xv, int64_t np, const float header[11]);
+void asm_free(void* p);
+/* The four files of write_mesh_separate (coarse_mesh.cpp:173-201) from a mesh
+ * that is already normalised: <base>n<inode>.dat, h, c (int32 nc + nc^3
+ * floats) and vel (int32 nc + 3 nc^3 floats). */
+int asm_write_mesh_separate(const char* base, int inode, const float* mesh, int nc);
+
+/* Timers (HIP events on the context stream) when ASM_OPT_PROFILE is set. */
+int asm_timer_count(void);
+const char* asm_timer_name(int i);
+/* Total milliseconds and number of launches of timer i since the last reset. */
+int asm_get_timer(asm_context* ctx, int i, double* total_ms, int64_t* launches);
+int asm_reset_timers(asm_context* ctx);
+/* Work counters of the last neighbour search: node tests, leaf scans, distance
+ * evaluations, list insertions. */
+int asm_get_search_stats(asm_context* ctx, uint64_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

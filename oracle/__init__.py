@@ -1,0 +1,285 @@
+"""ctypes bindings of the CPU oracle (liboracle.so) and of the compiled
+reference harness (_ref/libasmref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libasmref.so")
+K_SLOTS = 33
+
+_o = None
+_r = None
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def olib() -> C.CDLL:
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        L = C.CDLL(ORACLE_LIB)
+        vp, i32, i64, f32 = C.c_void_p, C.c_int, C.c_int64, C.c_float
+        P = C.POINTER
+        L.orc_buffered_set.restype = i64
+        L.orc_buffered_set.argtypes = [vp, i64, f32, f32, vp, i64, P(i64)]
+        for f in (L.orc_knn, L.orc_knn_brute):
+            f.restype = i32
+            f.argtypes = [vp, i64, vp, i64, i32, vp, vp, vp, i32]
+        L.orc_norm2.restype = f32
+        L.orc_norm2.argtypes = [vp, vp]
+        L.orc_tent.restype = f32
+        L.orc_tent.argtypes = [vp, vp, f32]
+        L.orc_density.restype = None
+        L.orc_density.argtypes = [vp, i64, vp, i32, vp, vp]
+        L.orc_ker_integ.restype = f32
+        L.orc_ker_integ.argtypes = [f32, f32]
+        L.orc_assign_on_mesh.restype = i32
+        L.orc_assign_on_mesh.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, vp]
+        L.orc_assign_on_global_mesh.restype = i32
+        L.orc_assign_on_global_mesh.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, vp, vp]
+        L.orc_mesh_total_f32.restype = None
+        L.orc_mesh_total_f32.argtypes = [vp, i32, vp]
+        L.orc_mesh_total_f64.restype = None
+        L.orc_mesh_total_f64.argtypes = [vp, i32, vp]
+        L.orc_normalize_f32.restype = None
+        L.orc_normalize_f32.argtypes = [vp, i32, f32]
+        L.orc_normalize_f64.restype = None
+        L.orc_normalize_f64.argtypes = [vp, i32, C.c_double]
+        L.orc_mark_halo.restype = None
+        L.orc_mark_halo.argtypes = [vp, i64, vp, i64, f32, f32, vp, P(i64), P(C.c_double)]
+        _o = L
+    return _o
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def rlib() -> C.CDLL:
+    global _r
+    if _r is None:
+        if not have_ref():
+            raise RuntimeError(f"{REF_LIB} missing (built only where the reference tree exists)")
+        L = C.CDLL(REF_LIB)
+        vp, i32, i64, f32, f64 = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_double
+        P = C.POINTER
+        L.ref_create.restype = vp
+        L.ref_create.argtypes = [vp, vp, vp, i32, i32, f32]
+        L.ref_destroy.restype = None
+        L.ref_destroy.argtypes = [vp]
+        L.ref_construct.restype = i32
+        L.ref_construct.argtypes = [vp, f32, f32, P(f64)]
+        L.ref_neighbor_search_all.restype = i32
+        L.ref_neighbor_search_all.argtypes = [vp, i32, P(f64)]
+        L.ref_knn_lists.restype = i32
+        L.ref_knn_lists.argtypes = [vp, vp, vp, vp]
+        L.ref_get_particles.restype = i32
+        L.ref_get_particles.argtypes = [vp, vp, vp, vp]
+        L.ref_set_dens.restype = i32
+        L.ref_set_dens.argtypes = [vp, vp, vp]
+        L.ref_assign_on_mesh.restype = i32
+        L.ref_assign_on_mesh.argtypes = [vp, vp, i32, i32, i32, vp, P(f64)]
+        L.ref_mark_halo.restype = i32
+        L.ref_mark_halo.argtypes = [vp, vp, i32, f32, f32, i32, P(i64), P(f64)]
+        L.ref_write_mesh_separate.restype = i32
+        L.ref_write_mesh_separate.argtypes = [C.c_char_p, i32, vp, i32, f32]
+        L.ref_time_path.restype = i32
+        L.ref_time_path.argtypes = [vp, vp, i32, i32, f32, f32, vp, i32, i32, vp, vp]
+        _r = L
+    return _r
+
+
+# ---------------------------------------------------------------- oracle ---
+
+def buffered_set(xv: np.ndarray, L: float, bf: float):
+    xv = np.ascontiguousarray(xv, np.float32).reshape(-1, 6)
+    cap = int(len(xv) * (1 + 2 * bf) ** 3 * 1.3) + 4096
+    out = np.empty((cap, 6), np.float32)
+    nl = C.c_int64()
+    n = olib().orc_buffered_set(_p(xv), len(xv), float(L), float(bf), _p(out), cap, C.byref(nl))
+    if n < 0:
+        raise RuntimeError("orc_buffered_set: capacity")
+    return out[:n].copy(), nl.value
+
+
+def knn(pos: np.ndarray, queries=None, k_slots: int = K_SLOTS, brute: bool = False, nthreads: int = 0):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    q = None if queries is None else np.ascontiguousarray(queries, np.int64)
+    nq = len(pos) if q is None else len(q)
+    nbr = np.empty((nq, k_slots), np.int32)
+    d2 = np.empty((nq, k_slots), np.float32)
+    rk = np.empty(nq, np.float32)
+    f = olib().orc_knn_brute if brute else olib().orc_knn
+    rc = f(_p(pos), len(pos), _p(q), nq, k_slots, _p(nbr), _p(d2), _p(rk), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"orc_knn failed ({rc})")
+    return nbr, d2, rk
+
+
+def density(pos: np.ndarray, nbr: np.ndarray, rk: np.ndarray) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    rk = np.ascontiguousarray(rk, np.float32)
+    d = np.zeros(len(pos), np.float64)
+    olib().orc_density(_p(pos), len(pos), _p(nbr), nbr.shape[1], _p(rk), _p(d))
+    return d
+
+
+def ker_integ(x1: float, x2: float) -> float:
+    return olib().orc_ker_integ(float(x1), float(x2))
+
+
+def tent(x, y, hinv) -> float:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return olib().orc_tent(_p(x), _p(y), float(hinv))
+
+
+def norm2(x, y) -> float:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return olib().orc_norm2(_p(x), _p(y))
+
+
+def assign_on_mesh(pos, vel, dens, rk, igrp, L: float, nc: int, sign: int = -1, periodic_coord=None):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    dens = np.ascontiguousarray(dens, np.float32)
+    rk = np.ascontiguousarray(rk, np.float32)
+    igrp = np.ascontiguousarray(igrp, np.int32)
+    mesh = np.zeros(6 * nc ** 3, np.float64)
+    if periodic_coord is None:
+        rc = olib().orc_assign_on_mesh(_p(pos), _p(vel), _p(dens), _p(rk), _p(igrp), len(pos), float(L),
+                                       int(nc), int(sign), _p(mesh))
+    else:
+        c3 = np.ascontiguousarray(periodic_coord, np.int32)
+        rc = olib().orc_assign_on_global_mesh(_p(pos), _p(vel), _p(dens), _p(rk), _p(igrp), len(pos),
+                                              float(L), int(nc), int(sign), _p(c3), _p(mesh))
+    if rc != 0:
+        raise RuntimeError(f"orc_assign_on_mesh failed ({rc})")
+    return mesh
+
+
+def normalize_f32(mesh: np.ndarray, nc: int, L: float) -> np.ndarray:
+    m = np.ascontiguousarray(mesh, np.float32).copy()
+    olib().orc_normalize_f32(_p(m), int(nc), float(L))
+    return m
+
+
+def normalize_f64(mesh: np.ndarray, nc: int, L: float) -> np.ndarray:
+    m = np.ascontiguousarray(mesh, np.float64).copy()
+    olib().orc_normalize_f64(_p(m), int(nc), float(L))
+    return m
+
+
+def mesh_total(mesh: np.ndarray, nc: int):
+    out = np.zeros(2, np.float64)
+    if mesh.dtype == np.float32:
+        olib().orc_mesh_total_f32(_p(np.ascontiguousarray(mesh)), int(nc), _p(out))
+    else:
+        olib().orc_mesh_total_f64(_p(np.ascontiguousarray(mesh, np.float64)), int(nc), _p(out))
+    return float(out[0]), float(out[1])
+
+
+def mark_halo(pos, halos, left: float, right: float, igrp=None):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    h = np.ascontiguousarray(halos, np.float32).reshape(-1, 5)
+    g = np.zeros(len(pos), np.int32) if igrp is None else np.ascontiguousarray(igrp, np.int32).copy()
+    nph, mh = C.c_int64(), C.c_double()
+    olib().orc_mark_halo(_p(pos), len(pos), _p(h), len(h), float(left), float(right), _p(g), C.byref(nph),
+                         C.byref(mh))
+    return g, nph.value, mh.value
+
+
+# ------------------------------------------------------------- reference ---
+
+class Ref:
+    """The reference's own functions on one particle set (single rank)."""
+
+    def __init__(self, pos, vel=None, igrp=None, np_local=None, boxsize: float = 1.0):
+        self.L = rlib()
+        self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        self.vel = None if vel is None else np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        self.igrp = None if igrp is None else np.ascontiguousarray(igrp, np.int32)
+        n = len(self.pos)
+        self.h = self.L.ref_create(_p(self.pos), _p(self.vel), _p(self.igrp), n,
+                                   n if np_local is None else int(np_local), float(boxsize))
+        self.n = n
+
+    def close(self):
+        if self.h:
+            self.L.ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def construct(self, left: float, right: float) -> int:
+        t = C.c_double()
+        return self.L.ref_construct(self.h, float(left), float(right), C.byref(t))
+
+    def neighbor_search_all(self, nthreads: int = 1) -> float:
+        t = C.c_double()
+        self.L.ref_neighbor_search_all(self.h, int(nthreads), C.byref(t))
+        return t.value
+
+    def knn_lists(self):
+        nbr = np.empty((self.n, K_SLOTS), np.int32)
+        d2 = np.empty((self.n, K_SLOTS), np.float32)
+        rk = np.empty(self.n, np.float32)
+        self.L.ref_knn_lists(self.h, _p(nbr), _p(d2), _p(rk))
+        return nbr, d2, rk
+
+    def particles(self):
+        rk = np.empty(self.n, np.float32)
+        dens = np.empty(self.n, np.float32)
+        igrp = np.empty(self.n, np.int32)
+        self.L.ref_get_particles(self.h, _p(rk), _p(dens), _p(igrp))
+        return rk, dens, igrp
+
+    def set_dens(self, dens=None, rk=None):
+        d = None if dens is None else np.ascontiguousarray(dens, np.float32)
+        r = None if rk is None else np.ascontiguousarray(rk, np.float32)
+        self.L.ref_set_dens(self.h, _p(d), _p(r))
+
+    def assign_on_mesh(self, nc: int, sign: int = -1, nthreads: int = 1):
+        mesh = np.empty(6 * nc ** 3, np.float32)
+        tot = np.zeros(2, np.float64)
+        t = C.c_double()
+        self.L.ref_assign_on_mesh(self.h, _p(mesh), int(nc), int(sign), int(nthreads), _p(tot), C.byref(t))
+        return mesh, tot
+
+    def mark_halo(self, halos, left: float, right: float, nthreads: int = 1):
+        h = np.ascontiguousarray(halos, np.float32).reshape(-1, 5)
+        nph, mh = C.c_int64(), C.c_double()
+        self.L.ref_mark_halo(self.h, _p(h), len(h), float(left), float(right), int(nthreads), C.byref(nph),
+                             C.byref(mh))
+        return nph.value, mh.value
+
+
+def ref_write_mesh_separate(base: str, inode: int, mesh: np.ndarray, nc: int, L: float) -> None:
+    m = np.ascontiguousarray(mesh, np.float32).copy()
+    rlib().ref_write_mesh_separate(base.encode(), int(inode), _p(m), int(nc), float(L))
+
+
+def ref_time_path(pos, vel, np_local: int, L: float, bf: float, ncs, nthreads: int):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    nc_arr = np.ascontiguousarray(ncs, np.int32)
+    sec = np.zeros(3, np.float64)
+    tot = np.zeros(2 * len(nc_arr), np.float64)
+    rlib().ref_time_path(_p(pos), _p(vel), len(pos), int(np_local), float(L), float(bf), _p(nc_arr), len(nc_arr),
+                         int(nthreads), _p(sec), _p(tot))
+    return sec, tot

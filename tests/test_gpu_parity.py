@@ -1,0 +1,155 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: bit-exact smoothing lengths and neighbour lists (canonical (d2, index)
+order); density and raw mesh cells within 1e-6 relative of the oracle's FP64
+sums of the reference's FP32 terms; integer work (halo marks) exact.
+"""
+import numpy as np
+import pytest
+
+import asmooth_amd as A
+import oracle as O
+from tests.helpers import gpu_path, mesh_fields, oracle_path, rel_err, small_set
+
+pytestmark = pytest.mark.gpu
+
+DENS_RTOL = 1e-6
+MESH_RTOL = 1e-6
+
+
+def _check_mesh(gm, om, nc, vel_scale):
+    g = mesh_fields(gm, nc)
+    o = mesh_fields(om, nc)
+    for f in ("n", "halo", "c"):
+        assert rel_err(g[f], o[f], floor=1e-30) <= MESH_RTOL, f
+    # velocity sums mix signs: error relative to the sum of |terms| <= n * max|v|
+    scale = np.repeat(o["n"], 3) * vel_scale + 1e-30
+    assert np.max(np.abs(g["v"] - o["v"]) / scale) <= MESH_RTOL
+
+
+@pytest.mark.parametrize("kind,n_side,seed", [("blobs", 16, 0), ("zeldovich", 16, 1), ("lattice", 10, 2),
+                                              ("uniform", 12, 3), ("blobs", 24, 4)])
+def test_path_matches_oracle(ctx, kind, n_side, seed):
+    xv, L, bf = small_set(kind, n_side, seed)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    ncs = [8, n_side]
+    o = oracle_path(buf, L, ncs)
+    g = gpu_path(ctx, parts, nl, L, bf, ncs)
+    # smoothing length and neighbour lists: bit-exact, including order
+    assert np.array_equal(g["rk"].view(np.int32), o["rk"].view(np.int32))
+    assert np.array_equal(g["nbr"], o["nbr"])
+    assert rel_err(g["dens"], o["dens64"]) <= DENS_RTOL
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    for nc in ncs:
+        _check_mesh(g["meshes"][nc], o["meshes"][nc], nc, vmax)
+        tn, th = g["totals"][nc]
+        on, oh = O.mesh_total(g["meshes"][nc], nc)
+        assert abs(tn - on) <= 1e-9 * abs(on) and th == oh
+
+
+@pytest.mark.parametrize("leaf_size", [8, 16, 64])
+def test_leaf_size_does_not_change_results(ctx, leaf_size):
+    xv, L, bf = small_set("blobs", 16, 7)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    nbr, _, rk = O.knn(buf[:, :3])
+    g = gpu_path(ctx, parts, nl, L, bf, [16], leaf_size=leaf_size)
+    assert np.array_equal(g["rk"], rk)
+    assert np.array_equal(g["nbr"], nbr)
+
+
+@pytest.mark.parametrize("tile", [1, 3, 5, 8])
+def test_tile_size_does_not_change_mesh(ctx, tile):
+    xv, L, bf = small_set("zeldovich", 12, 5)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    o = oracle_path(buf, L, [12, 7])
+    g = gpu_path(ctx, parts, nl, L, bf, [12, 7], tile=tile)
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    for nc in (12, 7):
+        _check_mesh(g["meshes"][nc], o["meshes"][nc], nc, vmax)
+
+
+def test_minimum_size_and_duplicates(ctx):
+    # 40 particles, several exactly coincident: d2 = 0 ties resolved by index
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(0, 8, (40, 3)).astype(np.float32)
+    pos[5] = pos[3]
+    pos[17] = pos[3]
+    pos[30] = pos[9]
+    parts = A.particles_from_xv(np.concatenate([pos, rng.standard_normal((40, 3)).astype(np.float32)], 1))
+    ctx.set_option(A.OPT_KEEP_NEIGHBORS, 1)
+    ctx.set_particles(parts, 40, 8.0)
+    ctx.construct(-0.4, 8.4)
+    ctx.neighbor_search(32)
+    rk, dens, _ = ctx.get_rk_dens()
+    nbr = ctx.get_neighbors()
+    onbr, _, ork = O.knn(pos)
+    assert np.array_equal(rk, ork)
+    assert np.array_equal(nbr, onbr)
+    assert rel_err(dens, O.density(pos, onbr, ork)) <= DENS_RTOL
+
+
+def test_too_few_particles_is_an_error(ctx):
+    parts = A.particles_from_xv(np.zeros((10, 6), np.float32))
+    with pytest.raises(A.AsmoothError):
+        ctx.set_particles(parts, 10, 1.0)
+
+
+def test_k_other_than_32_is_an_error(ctx):
+    xv, L, bf = small_set("uniform", 8, 1)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    ctx.set_particles(parts, nl, L)
+    ctx.construct(-bf * L, (1 + bf) * L)
+    with pytest.raises(A.AsmoothError):
+        ctx.neighbor_search(16)
+
+
+def test_halo_marking_matches_oracle(ctx):
+    xv, L, bf = small_set("blobs", 16, 9)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    rng = np.random.default_rng(5)
+    nh = 40
+    halos = np.zeros(nh, A.HALO_DTYPE)
+    halos["x"] = rng.uniform(-3, L + 3, (nh, 3)).astype(np.float32)
+    halos["r"] = rng.uniform(0.5, 4.0, nh).astype(np.float32)
+    halos["mass"] = rng.uniform(10, 1000, nh).astype(np.float32)
+    hflat = np.concatenate([halos["x"], halos["r"][:, None], halos["mass"][:, None]], 1)
+    og, onph, omh = O.mark_halo(buf[:, :3], hflat, -bf * L, L + bf * L)
+    o = oracle_path(buf, L, [16], igrp=og)
+    g = gpu_path(ctx, parts, nl, L, bf, [16], halos=halos)
+    assert np.array_equal(g["igrp"], og)
+    assert g["halo"][0] == onph
+    assert g["halo"][1] == pytest.approx(omh, rel=1e-12)
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    _check_mesh(g["meshes"][16], o["meshes"][16], 16, vmax)
+    assert mesh_fields(g["meshes"][16], 16)["halo"].sum() > 0
+
+
+def test_normalisation_matches_oracle_fp32(ctx):
+    xv, L, bf = small_set("zeldovich", 16, 3)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    gpu_path(ctx, parts, nl, L, bf, [16])
+    raw = ctx.get_mesh(16)
+    ctx.normalize_mesh(16)
+    got = ctx.get_mesh(16)
+    want = O.normalize_f32(raw, 16, L)
+    assert np.array_equal(got.view(np.int32), want.view(np.int32))
+
+
+def test_global_mesh_variant_matches_oracle(ctx):
+    xv, L, bf = small_set("blobs", 16, 12)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    o = oracle_path(buf, L, [])
+    gpu_path(ctx, parts, nl, L, bf, [])
+    for nc in (16, 10):
+        ctx.assign_on_global_mesh(nc, -1, (0, 0, 0))
+        gm = ctx.get_mesh(nc)
+        om = O.assign_on_mesh(buf[:, :3], buf[:, 3:], o["dens32"], o["rk"], np.zeros(len(buf), np.int32), L, nc,
+                              -1, periodic_coord=(0, 0, 0))
+        _check_mesh(gm, om, nc, float(np.max(np.abs(buf[:, 3:]))))
+        # periodic: every local particle's full weight lands on the mesh
+        assert abs(mesh_fields(gm, nc)["n"].sum() - nl) < 1e-3 * nl
