@@ -64,7 +64,7 @@ def _zeldovich_psi(n: int, L: float, rng: np.random.Generator, pk_index: float =
     del W, amp
     psi = []
     for kc in (kx, ky, kz):
-        psi.append(np.fft.irfftn(1j * kc * base, s=(n, n, n)).astype(np.float32))
+        psi.append(np.fft.irfftn(1j * kc * base, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32))
     return psi
 
 
