@@ -26,7 +26,7 @@ import numpy as np
 
 __all__ = [
     "PARTICLE_DTYPE", "HALO_DTYPE", "K_SLOTS", "AsmoothError", "Context", "lib",
-    "build_buffered_set", "build_node_share", "read_xv", "write_xv", "write_mesh_separate",
+    "build_buffered_set", "build_node_share", "build_buffered_halos", "read_halo_file", "halo_delta_vir", "read_xv", "write_xv", "write_mesh_separate",
     "particles_from_xv", "timer_names",
 ]
 
@@ -90,10 +90,15 @@ def lib() -> C.CDLL:
         "asm_get_neighbors": (i32, [vp, vp]),
         "asm_build_buffered_set": (i64, [vp, i64, f32, f32, vp, i64, P(i64)]),
         "asm_build_node_share": (i64, [vp, i64, f32, i32, P(i32), f32, vp, i64, P(i64)]),
+        "asm_build_buffered_halos": (i64, [vp, i64, f32, f32, vp, i64]),
+        "asm_read_halo_file": (i32, [C.c_char_p, f32, vp, f32, f32, i32, i32, P(vp), P(i64)]),
+        "asm_halo_delta_vir": (f64, [f64, f64]),
         "asm_read_xv": (i32, [C.c_char_p, P(vp), P(i64), vp]),
         "asm_write_xv": (i32, [C.c_char_p, vp, i64, vp]),
         "asm_free": (None, [vp]),
         "asm_write_mesh_separate": (i32, [C.c_char_p, i32, vp, i32]),
+        "asm_get_tree": (i32, [vp, vp, vp, P(i64), P(i64), P(i32)]),
+        "asm_get_sorted_results": (i32, [vp, vp, vp]),
         "asm_timer_count": (i32, []),
         "asm_timer_name": (C.c_char_p, [i32]),
         "asm_get_timer": (i32, [vp, i32, P(f64), P(i64)]),
@@ -240,6 +245,22 @@ class Context:
         self._check(self._L.asm_get_neighbors(self._h, _ptr(nbr)), "asm_get_neighbors")
         return nbr
 
+    def get_tree(self):
+        """(spos (n, 4) float32 with the original index in column 3 as int bits,
+        nodes (2*nleaf_pad, 8) int32 words, nleaf, nleaf_pad, leaf_size)."""
+        nl, npad, ls = C.c_int64(), C.c_int64(), C.c_int32()
+        self._check(self._L.asm_get_tree(self._h, None, None, C.byref(nl), C.byref(npad), C.byref(ls)), "asm_get_tree")
+        spos = np.empty((self.n, 4), np.float32)
+        nodes = np.empty((2 * npad.value, 8), np.int32)
+        self._check(self._L.asm_get_tree(self._h, _ptr(spos), _ptr(nodes), None, None, None), "asm_get_tree")
+        return spos, nodes, nl.value, npad.value, ls.value
+
+    def get_sorted_results(self):
+        rk = np.empty(self.n, np.float32)
+        d64 = np.empty(self.n, np.float64)
+        self._check(self._L.asm_get_sorted_results(self._h, _ptr(rk), _ptr(d64)), "asm_get_sorted_results")
+        return rk, d64
+
     # -- instrumentation -------------------------------------------------
     def timers(self) -> dict[str, tuple[float, int]]:
         out = {}
@@ -293,6 +314,38 @@ def build_node_share(xv: np.ndarray, box_total: float, nn: int, coord, bf: float
     if n < 0:
         raise AsmoothError(f"asm_build_node_share failed with status {n}")
     return out[:n].copy(), nl.value
+
+
+def build_buffered_halos(halos: np.ndarray, boxsize: float, bf: float) -> np.ndarray:
+    """Halo catalogue with the periodic images of a single rank (not trimmed)."""
+    h = np.ascontiguousarray(halos, HALO_DTYPE)
+    cap = 8 * len(h) + 16
+    out = np.empty(cap, HALO_DTYPE)
+    n = lib().asm_build_buffered_halos(_ptr(h) if len(h) else None, len(h), float(boxsize), float(bf), _ptr(out), cap)
+    if n < 0:
+        raise AsmoothError(f"asm_build_buffered_halos failed with status {n}")
+    return out[:n].copy()
+
+
+def read_halo_file(path: str, boxsize: float, shift=(0.0, 0.0, 0.0), z: float = 0.0, omega_m: float = 0.3,
+                   m200: bool = False, pid_flag: bool = True) -> np.ndarray:
+    """SO halo catalogue (22-float records, optional particle-ID blocks)."""
+    L = lib()
+    sh = np.ascontiguousarray(shift, np.float32)
+    buf, n = C.c_void_p(), C.c_int64()
+    rc = L.asm_read_halo_file(path.encode(), float(boxsize), _ptr(sh), float(z), float(omega_m), int(m200),
+                              int(pid_flag), C.byref(buf), C.byref(n))
+    if rc < 0:
+        raise AsmoothError(f"asm_read_halo_file({path}) failed with status {rc}")
+    try:
+        raw = np.ctypeslib.as_array(C.cast(buf, C.POINTER(C.c_float)), shape=(max(n.value, 1) * 5,)).copy()
+    finally:
+        L.asm_free(buf)
+    return raw[: n.value * 5].view(HALO_DTYPE).copy()
+
+
+def halo_delta_vir(z: float, omega_m: float) -> float:
+    return float(lib().asm_halo_delta_vir(float(z), float(omega_m)))
 
 
 def read_xv(path: str) -> tuple[np.ndarray, np.ndarray]:

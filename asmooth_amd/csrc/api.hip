@@ -621,6 +621,35 @@ int asm_get_neighbors(asm_context* ctx, int32_t* nbr)
     return ASM_OK;
 }
 
+int asm_get_tree(asm_context* ctx, float* spos, int32_t* nodes, int64_t* nleaf, int64_t* nleaf_pad,
+                 int32_t* leaf_size)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "no tree before asm_tree_construct");
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    if (spos) HIPC(hipMemcpy(spos, ctx->spos.p, sizeof(float4) * static_cast<size_t>(ctx->n), hipMemcpyDeviceToHost));
+    if (nodes)
+        HIPC(hipMemcpy(nodes, ctx->nodes.p, sizeof(Box) * static_cast<size_t>(2 * ctx->nleaf_pad),
+                       hipMemcpyDeviceToHost));
+    if (nleaf) *nleaf = ctx->nleaf;
+    if (nleaf_pad) *nleaf_pad = ctx->nleaf_pad;
+    if (leaf_size) *leaf_size = ctx->leaf_size;
+    return ASM_OK;
+}
+
+int asm_get_sorted_results(asm_context* ctx, float* rk, double* dens64)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "no search results");
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    if (rk) HIPC(hipMemcpy(rk, ctx->rk.p, sizeof(float) * static_cast<size_t>(ctx->n), hipMemcpyDeviceToHost));
+    if (dens64)
+        HIPC(hipMemcpy(dens64, ctx->dens64.p, sizeof(double) * static_cast<size_t>(ctx->n), hipMemcpyDeviceToHost));
+    return ASM_OK;
+}
+
 int asm_timer_count(void)
 {
     return T_COUNT;

@@ -1,5 +1,6 @@
 // Host-side steps either side of the device path: the buffered particle set
 // of a node, the cubep3m xv reader/writer and the mesh file writer.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -46,6 +47,59 @@ inline asm_particle make_rec(const float* u, const float* v)
     return p;
 }
 
+// Local records followed by their periodic images, in the order the
+// reference's single-rank exchange produces them (exchange_buffer.cpp:13-151,
+// exchange_buffer_template.h:18-122): candidates are taken in reverse file
+// order (the copy loop writes them from the end of the allocation down,
+// :36-47), cycle-sorted in place by region (:60-93), then sent per (source
+// region i, direction j) and shifted by -(dn-1) L (:110-151).  emit(rec,
+// shift) receives each record with the shift to subtract from its position
+// (zero for the local copies).
+template <typename Emit>
+void periodic_images(const float* recs, int width, int64_t n, float boxsize, float bf, Emit&& emit)
+{
+    const float a = bf * boxsize;            // exchange_buffer.cpp:23
+    const float b = (1.0f - bf) * boxsize;   // exchange_buffer.cpp:24
+    const float zero[3] = {0.0f, 0.0f, 0.0f};
+    std::vector<int64_t> q;
+    int64_t nbuf[27] = {0};
+    for (int64_t i = n - 1; i >= 0; --i) {
+        const int r = region_index(recs + width * i, a, b);
+        if (r != 13) {
+            q.push_back(i);
+            nbuf[r]++;
+        }
+    }
+    int64_t next[27], begin[27], end[27];
+    next[0] = begin[0] = 0;
+    end[0] = nbuf[0];
+    for (int i = 1; i < 27; ++i) {
+        next[i] = next[i - 1] + nbuf[i - 1];
+        begin[i] = next[i];
+        end[i] = next[i] + nbuf[i];
+    }
+    for (int i = 0; i < 27; ++i) {
+        for (int64_t j = next[i]; j < end[i]; ++j) {
+            int r = region_index(recs + width * q[j], a, b);
+            while (r != i) {
+                const int64_t k = next[r]++;
+                std::swap(q[k], q[j]);
+                r = region_index(recs + width * q[j], a, b);
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) emit(recs + width * i, zero);
+    for (int i = 0; i < 27; ++i) {
+        for (int j = 0; j < 27; ++j) {
+            if (!direction_carries(i, j)) continue;
+            const float s[3] = {static_cast<float>(j % 3 - 1) * boxsize,
+                                static_cast<float>((j / 3) % 3 - 1) * boxsize,
+                                static_cast<float>(j / 9 - 1) * boxsize};
+            for (int64_t k = begin[i]; k < begin[i] + nbuf[i]; ++k) emit(recs + width * q[k], s);
+        }
+    }
+}
+
 // remove_extra_particles (clumping_tree_main.cpp:550-576): keep records
 // strictly inside (-bf L, L + bf L)^3, order preserved.
 int64_t keep_inside(std::vector<asm_particle>& all, int64_t nlocal_in, float boxsize, float bf,
@@ -74,61 +128,34 @@ int64_t asm_build_buffered_set(const float* xv, int64_t np, float boxsize, float
                                asm_particle* out, int64_t cap, int64_t* np_local)
 {
     if (!xv || !out || np < 0 || !(boxsize > 0.0f) || !(bf > 0.0f)) return ASM_ERR_ARG;
-    const float a = bf * boxsize;                 // exchange_buffer.cpp:23
-    const float b = (1.0f - bf) * boxsize;        // exchange_buffer.cpp:24
-
-    // Buffer candidates in the order the reference's copy loop leaves them:
-    // written from the end of the allocation downwards, i.e. reverse file
-    // order (exchange_buffer.cpp:36-47).
-    std::vector<int64_t> q;
-    int64_t nbuf[27] = {0};
-    for (int64_t i = np - 1; i >= 0; --i) {
-        const int r = region_index(xv + 6 * i, a, b);
-        if (r != 13) {
-            q.push_back(i);
-            nbuf[r]++;
-        }
-    }
-    // The reference's in-place cycle sort by region (exchange_buffer.cpp:60-93),
-    // kept so that equal distances resolve to the same indices.
-    int64_t next[27], begin[27], end[27];
-    next[0] = begin[0] = 0;
-    end[0] = nbuf[0];
-    for (int i = 1; i < 27; ++i) {
-        next[i] = next[i - 1] + nbuf[i - 1];
-        begin[i] = next[i];
-        end[i] = next[i] + nbuf[i];
-    }
-    for (int i = 0; i < 27; ++i) {
-        for (int64_t j = next[i]; j < end[i]; ++j) {
-            int r = region_index(xv + 6 * q[j], a, b);
-            while (r != i) {
-                const int64_t k = next[r]++;
-                std::swap(q[k], q[j]);
-                r = region_index(xv + 6 * q[j], a, b);
-            }
-        }
-    }
-
     std::vector<asm_particle> all;
-    all.reserve(static_cast<size_t>(np + q.size() * 2));
-    for (int64_t i = 0; i < np; ++i) all.push_back(make_rec(xv + 6 * i, xv + 6 * i + 3));
-    // Images per (source region i, direction j) (exchange_buffer.cpp:110-151),
-    // shifted by -(dn-1) L (:138-144).
-    for (int i = 0; i < 27; ++i) {
-        for (int j = 0; j < 27; ++j) {
-            if (!direction_carries(i, j)) continue;
-            const float s[3] = {static_cast<float>(j % 3 - 1) * boxsize,
-                                static_cast<float>((j / 3) % 3 - 1) * boxsize,
-                                static_cast<float>(j / 9 - 1) * boxsize};
-            for (int64_t k = begin[i]; k < begin[i] + nbuf[i]; ++k) {
-                const float* p = xv + 6 * q[k];
-                const float u[3] = {p[0] - s[0], p[1] - s[1], p[2] - s[2]};
-                all.push_back(make_rec(u, p + 3));
-            }
-        }
-    }
+    periodic_images(xv, 6, np, boxsize, bf, [&](const float* rec, const float* shift) {
+        const float u[3] = {rec[0] - shift[0], rec[1] - shift[1], rec[2] - shift[2]};
+        all.push_back(make_rec(u, rec + 3));
+    });
     return keep_inside(all, np, boxsize, bf, out, cap, np_local);
+}
+
+int64_t asm_build_buffered_halos(const asm_halo* halos, int64_t n, float boxsize, float bf, asm_halo* out,
+                                 int64_t cap)
+{
+    if ((!halos && n > 0) || !out || n < 0 || !(boxsize > 0.0f) || !(bf > 0.0f)) return ASM_ERR_ARG;
+    static_assert(sizeof(asm_halo) == 5 * sizeof(float), "halo record is 5 floats");
+    int64_t cnt = 0;
+    bool overflow = false;
+    periodic_images(reinterpret_cast<const float*>(halos), 5, n, boxsize, bf,
+                    [&](const float* rec, const float* shift) {
+                        if (cnt >= cap) {
+                            overflow = true;
+                            return;
+                        }
+                        asm_halo h;
+                        for (int a = 0; a < 3; ++a) h.x[a] = rec[a] - shift[a];
+                        h.r = rec[3];
+                        h.mass = rec[4];
+                        out[cnt++] = h;
+                    });
+    return overflow ? ASM_ERR_CAPACITY : cnt;
 }
 
 int64_t asm_build_node_share(const float* xv, int64_t np, float box_total, int nn, const int coord[3],
@@ -214,6 +241,66 @@ int asm_write_xv(const char* path, const float* xv, int64_t np, const float head
               std::fwrite(xv, sizeof(float), 6 * static_cast<size_t>(np), fp) == 6 * static_cast<size_t>(np);
     ok = (std::fclose(fp) == 0) && ok;
     return ok ? ASM_OK : ASM_ERR_IO;
+}
+
+
+// Bryan & Norman virial overdensity relative to the mean (halo_file2.cpp:47-61).
+double asm_halo_delta_vir(double z, double omega_m)
+{
+    const double a = 1.0 / (1.0 + z);
+    const double omega_l = 1.0 - omega_m;
+    const double omega_z = omega_m / (omega_m + omega_l * a * a * a);
+    const double x = omega_z - 1.0;
+    return (18.0 * M_PI * M_PI + 82.0 * x - 39.0 * x * x) * ((omega_m + omega_l * a * a * a) / omega_m);
+}
+
+int asm_read_halo_file(const char* path, float boxsize, const float shift[3], float z, float omega_m, int m200,
+                       int pid_flag, asm_halo** halos, int64_t* n_halo)
+{
+    if (!path || !halos || !n_halo || !(boxsize > 0.0f)) return ASM_ERR_ARG;
+    // Record layout (halo_file2.cpp:17-30): 22 floats; position at 0..2; mass
+    // at 3 and radius at 5 (Bryan-Norman), or 4 and 6 (M200).
+    const int ndat = 22, off_m = m200 ? 4 : 3, off_r = m200 ? 6 : 5;
+    if (!m200 && !(omega_m > 0.0f)) return ASM_ERR_ARG;
+    const float delta = m200 ? 200.0f : static_cast<float>(asm_halo_delta_vir(z, omega_m));
+    const float torr = 10.0f;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return ASM_ERR_IO;
+    int32_t header[3];
+    if (std::fread(header, sizeof(int32_t), 3, fp) != 3) {
+        std::fclose(fp);
+        return ASM_ERR_IO;
+    }
+    std::vector<asm_halo> out;
+    float buf[22];
+    const float sh[3] = {shift ? shift[0] : 0.0f, shift ? shift[1] : 0.0f, shift ? shift[2] : 0.0f};
+    while (std::fread(buf, sizeof(float), ndat, fp) == static_cast<size_t>(ndat)) {
+        asm_halo h;
+        for (int a = 0; a < 3; ++a) h.x[a] = std::fmod(buf[a], boxsize) - sh[a];
+        const float r = buf[off_r];
+        const float m = buf[off_m];
+        h.r = r;
+        h.mass = m;
+        // overdensity sanity check (halo_file2.cpp:114-121), an error here
+        if (std::fabs(m / (4.0f / 3.0f * M_PI * r * r * r) - delta) >= torr) {
+            std::fclose(fp);
+            return ASM_ERR_IO;
+        }
+        // 50 particle IDs (8 bytes) and their x,v (6 floats) follow each record
+        // when the catalogue carries them (halo_file2.cpp:123-127).
+        if (pid_flag && std::fseek(fp, 50 * (8 + 4 * 6), SEEK_CUR) != 0) {
+            std::fclose(fp);
+            return ASM_ERR_IO;
+        }
+        out.push_back(h);
+    }
+    std::fclose(fp);
+    asm_halo* mem = static_cast<asm_halo*>(std::malloc(sizeof(asm_halo) * (out.empty() ? 1 : out.size())));
+    if (!mem) return ASM_ERR_CAPACITY;
+    if (!out.empty()) std::memcpy(mem, out.data(), sizeof(asm_halo) * out.size());
+    *halos = mem;
+    *n_halo = static_cast<int64_t>(out.size());
+    return ASM_OK;
 }
 
 void asm_free(void* p)

@@ -73,6 +73,7 @@ __global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint32_t* __res
     __shared__ uint32_t cnt[kRadix * kCntStride];
     __shared__ uint32_t wsum[kSortThreads / kWave];
     __shared__ uint32_t gbase[kRadix];
+    __shared__ int64_t gofs[kRadix];
     const int t = threadIdx.x;
 #pragma unroll
     for (int d = 0; d < kRadix; ++d) cnt[d * kCntStride + t] = 0u;
@@ -128,15 +129,18 @@ __global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint32_t* __res
         run += s[i];
     }
     __syncthreads();
-    // cnt now holds the in-tile start of each (digit, thread) run.  The tile's
-    // digit-d keys start at gbase[d] - (in-tile start of digit d, thread 0).
+    // cnt now holds the in-tile start of each (digit, thread) run.  A key of
+    // digit d goes to gbase[d] + (its in-tile rank among digit-d keys), i.e.
+    // gofs[d] + cnt[d][t] with gofs[d] = gbase[d] - (in-tile start of digit d),
+    // taken before any thread advances its own counter.
+    if (t < kRadix) gofs[t] = static_cast<int64_t>(gbase[t]) - static_cast<int64_t>(cnt[t * kCntStride]);
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         if (j >= nvalid) break;
         const uint32_t d = (k[j] >> shift) & (kRadix - 1);
-        const uint32_t local = cnt[d * kCntStride + t] - cnt[d * kCntStride];
+        const int64_t pos = gofs[d] + cnt[d * kCntStride + t];
         cnt[d * kCntStride + t] += 1u;
-        const int64_t pos = static_cast<int64_t>(gbase[d]) + local;
         kout[pos] = k[j];
         vout[pos] = v[j];
     }

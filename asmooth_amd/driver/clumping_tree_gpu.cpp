@@ -39,7 +39,8 @@ std::map<std::string, std::string> parse_options(int argc, char** argv)
     std::map<std::string, std::string> op = {
         {"-pm_redshift", "redshifts.txt"}, {"-node_dir", "results/node"}, {"-allocate", "512"},
         {"-xv", "-1"}, {"-buffer_factor", "0.05"}, {"-nc_node_dim", "0"}, {"-l", "0.2"},
-        {"-nc", "256"}, {"-mesh_scale", "4"}, {"-omegam", "0"}, {"-device", "0"}, {"-halo_excise", "0"}};
+        {"-nc", "256"}, {"-mesh_scale", "4"}, {"-omegam", "0"}, {"-device", "0"}, {"-halo_excise", "0"},
+        {"-m200", "0"}, {"-pid_flag", "1"}};
     for (int j = 1; j + 1 < argc; j += 2) {
         if (argv[j][0] != '-')
             std::fprintf(stderr, "Error: %s must be an option\n", argv[j]);
@@ -96,7 +97,10 @@ int main(int argc, char** argv)
                     "\t-xv <local boxsize>\n"
                     "\t-nc <nc1,nc2,...> list of number of mesh\n"
                     "\t-device <hip device ordinal>\n"
-                    "\t-halo_excise 1 (read <node_dir>0/<z>halo0.dat as {x,y,z,r,mass} records)\n");
+                    "\t-halo_excise 1 (excise SO halos of <node_dir>0/<z>halo0.dat)\n"
+                    "\t-omegam 0.308; (for Bryan-Norman over density)\n"
+                    "\t-m200 0|1 (M200/r200 columns instead of Bryan-Norman)\n"
+                    "\t-pid_flag 0|1 (halo records followed by particle-ID blocks; default 1)\n");
         return 0;
     }
     auto op = parse_options(argc, argv);
@@ -202,27 +206,36 @@ int main(int argc, char** argv)
         double t2 = now_s();
 
         if (std::atoi(op["-halo_excise"].c_str())) {
+            // read_and_exchage_halo (halo_file2.cpp:63-86) + the marking loop
+            // (clumping_tree_main.cpp:452-511)
             char hf[512];
             std::snprintf(hf, sizeof(hf), "%s%d/%shalo%d.dat", node_dir.c_str(), inode, z.c_str(), inode);
-            FILE* fp = std::fopen(hf, "rb");
-            if (!fp) {
-                std::fprintf(stderr, "Unable to open halo data: %s\n", hf);
+            asm_halo* halos = nullptr;
+            int64_t nh = 0;
+            const float omegam = static_cast<float>(std::atof(op["-omegam"].c_str()));
+            if ((rc = asm_read_halo_file(hf, boxsize, shift, static_cast<float>(std::atof(z.c_str())), omegam,
+                                         std::atoi(op["-m200"].c_str()), std::atoi(op["-pid_flag"].c_str()),
+                                         &halos, &nh))) {
+                std::fprintf(stderr, "Unable to read halo data: %s (%d)\n", hf, rc);
                 return 1;
             }
-            std::vector<asm_halo> halos;
-            asm_halo h;
-            while (std::fread(&h, sizeof(asm_halo), 1, fp) == 1) {
-                for (int a = 0; a < 3; ++a) h.x[a] -= shift[a];
-                halos.push_back(h);
-            }
-            std::fclose(fp);
+            std::vector<asm_halo> hbuf(static_cast<size_t>(8 * nh + 16));
+            const int64_t nhb = asm_build_buffered_halos(halos, nh, boxsize, bf, hbuf.data(),
+                                                         static_cast<int64_t>(hbuf.size()));
+            asm_free(halos);
+            if (nhb < 0) return die(ctx, "asm_build_buffered_halos", static_cast<int>(nhb));
+            std::printf("nhalo_local= %lld\n", static_cast<long long>(nh));
             int64_t np_halo = 0;
             double m_halo = 0.0;
-            if ((rc = asm_mark_halo_particles(ctx, halos.data(), static_cast<int64_t>(halos.size()), -boxsize * bf,
-                                              boxsize + boxsize * bf, &np_halo, &m_halo)))
+            if ((rc = asm_mark_halo_particles(ctx, hbuf.data(), nhb, -boxsize * bf, boxsize + boxsize * bf,
+                                              &np_halo, &m_halo)))
                 return die(ctx, "asm_mark_halo_particles", rc);
+            std::printf("n_so_halo %lld %lld\n", static_cast<long long>(nh), static_cast<long long>(nhb));
             std::printf("so_halo_local %lld %g\n", static_cast<long long>(np_halo), m_halo);
             std::printf("check halo_mass %s %lld %g\n", z.c_str(), static_cast<long long>(np_halo), m_halo);
+            if (m_halo > 100.0 && !(0.80 < np_halo / m_halo && np_halo / m_halo < 1.20))
+                std::fprintf(stderr, "Warning: halo particle count / mass ratio %g outside (0.8, 1.2)\n",
+                             np_halo / m_halo);
         } else {
             std::printf("haloes not excised\n");
         }

@@ -136,6 +136,15 @@ int asm_get_rk_dens(asm_context* ctx, float* rk, float* dens, int32_t* igrp);
  * kept by the last search when ASM_OPT_KEEP_NEIGHBORS is set. */
 int asm_get_neighbors(asm_context* ctx, int32_t* nbr);
 
+/* The device tree as built, for inspection: sorted positions (n x 4 floats:
+ * x, y, z, original index as int32 bits) and node records ((2*nleaf_pad) x 8
+ * words: lo xyz, first particle, hi xyz, particle count; node 1 is the root,
+ * node i has children 2i and 2i+1, leaves are nleaf_pad .. 2*nleaf_pad-1). */
+int asm_get_tree(asm_context* ctx, float* spos, int32_t* nodes, int64_t* nleaf, int64_t* nleaf_pad,
+                 int32_t* leaf_size);
+/* Search results in sorted (tree) order: rk and the FP64 density sums. */
+int asm_get_sorted_results(asm_context* ctx, float* rk, double* dens64);
+
 /* Host steps either side of the path. */
 /* Buffered set of a single periodic rank: the 26 periodic images of
  * exchange_buffer (exchange_buffer.cpp:13-167, one rank) followed by
@@ -149,6 +158,20 @@ int64_t asm_build_buffered_set(const float* xv, int64_t np, float boxsize, float
 int64_t asm_build_node_share(const float* xv, int64_t np, float box_total, int nn,
                              const int coord[3], float bf, asm_particle* out, int64_t cap,
                              int64_t* np_local);
+/* Periodic images of a single rank's halo catalogue (exchange_buffer_template.h:
+ * 18-122 with one rank; halos are not trimmed).  Returns the count. */
+int64_t asm_build_buffered_halos(const asm_halo* halos, int64_t n, float boxsize, float bf,
+                                 asm_halo* out, int64_t cap);
+/* Halo catalogue reader (halo_file2.cpp:88-138): 3 x int32 header, then
+ * records of 22 float32 (position at 0..2, mass/radius at 3/5, or 4/6 with
+ * m200); positions are taken modulo boxsize minus shift; records whose
+ * overdensity differs from Delta (Bryan-Norman at z, omega_m, or 200) by 10
+ * or more are an error; pid_flag skips the 50 x 32-byte particle block after
+ * each record.  *halos is malloc'ed; free with asm_free. */
+int asm_read_halo_file(const char* path, float boxsize, const float shift[3], float z, float omega_m,
+                       int m200, int pid_flag, asm_halo** halos, int64_t* n_halo);
+/* Bryan & Norman (1998) virial overdensity (halo_file2.cpp:47-61). */
+double asm_halo_delta_vir(double z, double omega_m);
 /* cubep3m xv reader (file_reader2.cpp:24-86): header = int32 np + 11 float32,
  * then np x {x,y,z,vx,vy,vz}.  *xv is malloc'ed; free with asm_free. */
 int asm_read_xv(const char* path, float** xv, int64_t* np, float header[11]);

@@ -153,3 +153,52 @@ def test_global_mesh_variant_matches_oracle(ctx):
         _check_mesh(gm, om, nc, float(np.max(np.abs(buf[:, 3:]))))
         # periodic: every local particle's full weight lands on the mesh
         assert abs(mesh_fields(gm, nc)["n"].sum() - nl) < 1e-3 * nl
+
+
+def _morton(pos, left, right):
+    scale = np.float32(1024.0) / (np.float32(right) - np.float32(left))
+    c = np.clip((pos.astype(np.float32) - np.float32(left)) * scale, np.float32(0), np.float32(1023)).astype(np.uint32)
+
+    def spread(v):
+        v = v & 0x3FF
+        v = (v | (v << 16)) & 0x030000FF
+        v = (v | (v << 8)) & 0x0300F00F
+        v = (v | (v << 4)) & 0x030C30C3
+        v = (v | (v << 2)) & 0x09249249
+        return v
+    return spread(c[:, 0]) | (spread(c[:, 1]) << 1) | (spread(c[:, 2]) << 2)
+
+
+@pytest.mark.parametrize("kind,n_side,leaf", [("blobs", 20, 32), ("zeldovich", 24, 16), ("lattice", 12, 64)])
+def test_tree_invariants(ctx, kind, n_side, leaf):
+    """Stable Morton sort and a balanced tree of exact bounding boxes over a
+    partition of the buffered set (the tree shape itself is free)."""
+    xv, L, bf = small_set(kind, n_side, 3)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    n = len(parts)
+    ctx.set_option(A.OPT_LEAF_SIZE, leaf)
+    ctx.set_particles(parts, nl, L)
+    left, right = -bf * L, (1 + bf) * L
+    ctx.construct(left, right)
+    spos, nodes, nleaf, npad, ls = ctx.get_tree()
+    perm = spos[:, 3].view(np.int32)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert np.array_equal(spos[:, :3], parts["x"][perm])
+    keys = _morton(spos[:, :3], left, right)
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    same = np.diff(keys.astype(np.int64)) == 0
+    assert np.all(np.diff(perm)[same] > 0)          # stable: ties keep input order
+    assert nleaf == (n + leaf - 1) // leaf and ls == leaf and npad >= nleaf
+    boxes = nodes.view(np.float32)
+    for j in range(nleaf):
+        lo, hi = j * leaf, min(j * leaf + leaf, n)
+        b = boxes[npad + j]
+        assert nodes[npad + j, 3] == lo and nodes[npad + j, 7] == hi - lo
+        assert np.array_equal(b[:3], spos[lo:hi, :3].min(0)) and np.array_equal(b[4:7], spos[lo:hi, :3].max(0))
+    for i in range(1, npad):
+        a, c = boxes[2 * i], boxes[2 * i + 1]
+        assert np.array_equal(boxes[i, :3], np.minimum(a[:3], c[:3]))
+        assert np.array_equal(boxes[i, 4:7], np.maximum(a[4:7], c[4:7]))
+        assert nodes[i, 7] == nodes[2 * i, 7] + nodes[2 * i + 1, 7]
+    assert nodes[1, 7] == n
+    ctx.set_option(A.OPT_LEAF_SIZE, 32)
