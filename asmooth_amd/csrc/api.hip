@@ -366,7 +366,8 @@ int asm_neighbor_search(asm_context* ctx, int k)
         StageTimer t(ctx, T_KNN);
         HIPC(launch_fill_zero_f64(ctx->dens64.p, n, s));
         HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->kstats.p), sizeof(KnnStats) / 4, s));
-        HIPC(launch_knn_dens(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->inv_perm.p, ctx->rk.p,
+        HIPC(launch_knn_dens(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->nleaf, ctx->leaf_size,
+                             ctx->inv_perm.p, ctx->rk.p,
                              ctx->dens64.p, nbr, ctx->kstats.p, s));
         HIPC(launch_dens_finalize(ctx->dens64.p, n, ctx->dens.p, s));
     }
@@ -383,7 +384,7 @@ int asm_mark_halo_particles(asm_context* ctx, const asm_halo* halos, int64_t n_h
     HIPC(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     HIPC(ctx->halo_count.ensure(1));
-    HIPC(hipMemsetAsync(ctx->halo_count.p, 0, sizeof(unsigned long long), s));
+    HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->halo_count.p), 2, s));
     double mass = 0.0;
     for (int64_t i = 0; i < n_halo; ++i) {
         const asm_halo& h = halos[i];
@@ -436,6 +437,7 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     a.coord[1] = coord ? coord[1] : 0;
     a.coord[2] = coord ? coord[2] : 0;
     const int64_t ntiles = static_cast<int64_t>(a.ntx) * a.ntx * a.ntx;
+    if (ntiles > 0x7fffffffLL) return fail(ctx, ASM_ERR_CAPACITY, "nc %d with tile %d gives too many tiles", nc, a.tile);
     HIPC(ctx->tile_cnt.ensure(ntiles));
     HIPC(ctx->tile_start.ensure(ntiles));
     HIPC(ctx->tile_cur.ensure(ntiles));

@@ -89,56 +89,136 @@ __device__ __forceinline__ PartRange part_range(const DepositArgs& a, int64_t i)
     return r;
 }
 
-__device__ __forceinline__ int floor_div(int a, int b)
-{
-    return (a >= 0) ? a / b : -((-a + b - 1) / b);
-}
+// Tiles whose cells a particle's range [lo, hi) touches, per axis, as up to
+// three disjoint tile intervals.  Node mesh: one interval.  Periodic variant:
+// the unwrapped range is cut at multiples of nc, each piece is wrapped into
+// [0, nc) and mapped to tiles, and overlapping intervals are merged so that
+// no tile is listed twice (nc need not be a multiple of the tile side).
+struct AxisTiles {
+    int a[3], b[3];  // inclusive tile intervals
+    int n;
+};
 
-// Tile range of one axis for cell range [lo, hi).  Periodic ranges can run
-// past [0, nc); they are split by wrapping, so a tile index is taken modulo
-// the tile count and a range is limited to at most one full turn.
-template <typename F>
-__device__ __forceinline__ void for_each_tile(const DepositArgs& a, const PartRange& r, F&& f)
+__device__ __forceinline__ AxisTiles axis_tiles(const DepositArgs& a, int lo, int hi)
 {
-    int t0[3], t1[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        t0[d] = floor_div(r.lo[d], a.tile);
-        t1[d] = floor_div(r.hi[d] - 1, a.tile);
-        if (a.periodic && t1[d] - t0[d] >= a.ntx) t1[d] = t0[d] + a.ntx - 1;
+    AxisTiles t;
+    t.n = 0;
+    if (!a.periodic) {
+        t.a[0] = lo / a.tile;
+        t.b[0] = (hi - 1) / a.tile;
+        t.n = 1;
+        return t;
     }
-    for (int tz = t0[2]; tz <= t1[2]; ++tz) {
-        const int wz = ((tz % a.ntx) + a.ntx) % a.ntx;
-        for (int ty = t0[1]; ty <= t1[1]; ++ty) {
-            const int wy = ((ty % a.ntx) + a.ntx) % a.ntx;
-            for (int tx = t0[0]; tx <= t1[0]; ++tx) {
-                const int wx = ((tx % a.ntx) + a.ntx) % a.ntx;
-                f((static_cast<int64_t>(wz) * a.ntx + wy) * a.ntx + wx);
+    for (int k = -1; k <= 1; ++k) {
+        const int s0 = max(lo, k * a.nc), s1 = min(hi, (k + 1) * a.nc);
+        if (s0 >= s1) continue;
+        int ta = (s0 - k * a.nc) / a.tile;
+        int tb = (s1 - 1 - k * a.nc) / a.tile;
+        // merge with any interval it overlaps or touches
+        for (int m = 0; m < t.n;) {
+            if (ta <= t.b[m] + 1 && t.a[m] <= tb + 1) {
+                ta = min(ta, t.a[m]);
+                tb = max(tb, t.b[m]);
+                t.a[m] = t.a[t.n - 1];
+                t.b[m] = t.b[t.n - 1];
+                --t.n;
+                m = 0;
+            } else {
+                ++m;
             }
         }
+        t.a[t.n] = ta;
+        t.b[t.n] = tb;
+        ++t.n;
+    }
+    return t;
+}
+
+__device__ __forceinline__ int axis_count(const AxisTiles& t)
+{
+    int c = 0;
+    for (int m = 0; m < t.n; ++m) c += t.b[m] - t.a[m] + 1;
+    return c;
+}
+
+__device__ __forceinline__ int axis_at(const AxisTiles& t, int k)
+{
+    for (int m = 0; m < t.n; ++m) {
+        const int len = t.b[m] - t.a[m] + 1;
+        if (k < len) return t.a[m] + k;
+        k -= len;
+    }
+    return 0;
+}
+
+// Add 1 to counters[t] for every active lane, one atomic per distinct t in
+// the wavefront (Morton-sorted particles share tiles, so per-lane atomics on
+// the same counter would serialise).  Each active lane gets its own slot:
+// the counter's old value plus its rank among the lanes sharing t.
+__device__ __forceinline__ uint32_t wave_aggregated_inc(uint32_t* counters, int t, bool active)
+{
+    const int lane = threadIdx.x % kWave;
+    uint64_t pending = __ballot(active);
+    uint32_t mine = 0;
+    while (pending) {
+        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+        const int tl = __shfl(t, leader);
+        const uint64_t grp = pending & __ballot(t == tl);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counters[tl], static_cast<uint32_t>(__popcll(grp)));
+        base = __shfl(base, leader);
+        if ((grp >> lane) & 1ull) mine = base + static_cast<uint32_t>(__popcll(grp & ((1ull << lane) - 1ull)));
+        pending &= ~grp;
+    }
+    return mine;
+}
+
+// Count (list == nullptr) or fill the per-tile particle lists.  Every lane of
+// a wavefront walks the same number of (particle, tile) steps so the
+// aggregated atomics see the whole wavefront.
+__device__ __forceinline__ void tile_pass(const DepositArgs& a, uint32_t* counters, const uint32_t* tile_start,
+                                          uint32_t* list)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool active = i < a.n;
+    PartRange r;
+    AxisTiles tx, ty, tz;
+    int cx = 0, cy = 0, total = 0;
+    if (active) {
+        r = part_range(a, i);
+        active = r.active;
+    }
+    if (active) {
+        tx = axis_tiles(a, r.lo[0], r.hi[0]);
+        ty = axis_tiles(a, r.lo[1], r.hi[1]);
+        tz = axis_tiles(a, r.lo[2], r.hi[2]);
+        cx = axis_count(tx);
+        cy = axis_count(ty);
+        total = cx * cy * axis_count(tz);
+    }
+    int maxk = total;
+    for (int o = kWave / 2; o > 0; o >>= 1) maxk = max(maxk, __shfl_xor(maxk, o));
+    for (int k = 0; k < maxk; ++k) {
+        const bool act = k < total;
+        int t = 0;
+        if (act) {
+            const int kx = k % cx, ky = (k / cx) % cy, kz = k / (cx * cy);
+            t = (axis_at(tz, kz) * a.ntx + axis_at(ty, ky)) * a.ntx + axis_at(tx, kx);
+        }
+        const uint32_t slot = wave_aggregated_inc(counters, t, act);
+        if (list && act) list[tile_start[t] + slot] = static_cast<uint32_t>(i);
     }
 }
 
 __global__ void deposit_count_kernel(DepositArgs a, uint32_t* __restrict__ tile_cnt)
 {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const PartRange r = part_range(a, i);
-    if (!r.active) return;
-    for_each_tile(a, r, [&](int64_t t) { atomicAdd(&tile_cnt[t], 1u); });
+    tile_pass(a, tile_cnt, nullptr, nullptr);
 }
 
 __global__ void deposit_fill_kernel(DepositArgs a, const uint32_t* __restrict__ tile_start,
                                     uint32_t* __restrict__ tile_cur, uint32_t* __restrict__ list)
 {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const PartRange r = part_range(a, i);
-    if (!r.active) return;
-    for_each_tile(a, r, [&](int64_t t) {
-        const uint32_t slot = atomicAdd(&tile_cur[t], 1u);
-        list[tile_start[t] + slot] = static_cast<uint32_t>(i);
-    });
+    tile_pass(a, tile_cur, tile_start, list);
 }
 
 // One workgroup per tile: FP64 LDS accumulation, then one flush of the tile.

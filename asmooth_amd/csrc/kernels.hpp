@@ -43,7 +43,7 @@ struct KnnStats {
     unsigned long long inserts;
 };
 hipError_t launch_knn_dens(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad,
-                           const int32_t* inv_perm, float* rk_sorted, double* dens64,
+                           int64_t nleaf, int leaf_size, const int32_t* inv_perm, float* rk_sorted, double* dens64,
                            int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s);
 // dens (sorted order, float) from the FP64 sums.
 hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s);

@@ -62,9 +62,37 @@ __device__ __forceinline__ void wave_lds_fence()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Every lane tests particles [start, start + cnt) of the sorted array,
+// staged through LDS 64 at a time.
+template <int K>
+__device__ __forceinline__ void scan_range(const float4* __restrict__ spos, int64_t start, int64_t cnt,
+                                           float4* stage, int lane, float qx, float qy, float qz,
+                                           uint64_t (&key)[K], float& kth, unsigned long long& n_ins)
+{
+    for (int64_t base = 0; base < cnt; base += kWave) {
+        const int m = static_cast<int>(cnt - base < kWave ? cnt - base : kWave);
+        if (lane < m) stage[lane] = spos[start + base + lane];
+        wave_lds_fence();
+        for (int j = 0; j < m; ++j) {
+            const float4 c = stage[j];
+            const float d2 = norm2(qx, qy, qz, c.x, c.y, c.z);
+            if (d2 <= kth) {
+                const uint64_t ck = make_key(d2, __float_as_uint(c.w));
+                if (ck < key[K - 1]) {
+                    sorted_insert<K>(key, ck);
+                    kth = key_d2(key[K - 1]);
+                    ++n_ins;
+                }
+            }
+        }
+        wave_lds_fence();
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad,
+    int64_t nleaf, int leaf_size, int init_leaves,
     const int32_t* __restrict__ inv_perm, float* __restrict__ rk_sorted, double* __restrict__ dens64,
     int32_t* __restrict__ nbr_out, KnnStats* __restrict__ stats)
 {
@@ -86,6 +114,23 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     unsigned long long n_node = 0, n_leaf = 0, n_dist = 0, n_ins = 0;
     const unsigned long long n_valid_lanes = __popcll(__ballot(valid));
 
+    // Seed every lane's list from the wavefront's own leaves and init_leaves
+    // leaves on either side in Morton order: a compact neighbourhood, so the
+    // k-th distances are close to final before the tree walk starts, and the
+    // walk then opens few nodes.  These leaves are skipped by the walk.
+    const int64_t last = (n < wave0 + kWave ? n : wave0 + kWave) - 1;
+    int64_t l0 = wave0 / leaf_size - init_leaves;
+    int64_t l1 = last / leaf_size + init_leaves;
+    if (l0 < 0) l0 = 0;
+    if (l1 > nleaf - 1) l1 = nleaf - 1;
+    {
+        const int64_t p0 = l0 * leaf_size;
+        const int64_t p1 = ((l1 + 1) * leaf_size < n) ? (l1 + 1) * leaf_size : n;
+        scan_range<K>(spos, p0, p1 - p0, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
+        n_leaf += static_cast<unsigned long long>(l1 - l0 + 1);
+        n_dist += static_cast<unsigned long long>(p1 - p0);
+    }
+
     int stack = 0;
     int sp = 1;
     stack = (lane == 0) ? 1 : stack;
@@ -96,25 +141,11 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
         ++n_node;
         if (!wave_any(valid && box_d2(b, qx, qy, qz) <= kth)) continue;
         if (node >= nleaf_pad) {
-            const int start = b.lo_w;
-            const int cnt = b.hi_w;
+            const int64_t leaf = node - nleaf_pad;
+            if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
             ++n_leaf;
-            n_dist += static_cast<unsigned long long>(cnt);
-            if (lane < cnt) stage[wid][lane] = spos[start + lane];
-            wave_lds_fence();
-            for (int j = 0; j < cnt; ++j) {
-                const float4 c = stage[wid][j];
-                const float d2 = norm2(qx, qy, qz, c.x, c.y, c.z);
-                if (d2 <= kth) {
-                    const uint64_t ck = make_key(d2, __float_as_uint(c.w));
-                    if (ck < key[K - 1]) {
-                        sorted_insert<K>(key, ck);
-                        kth = key_d2(key[K - 1]);
-                        ++n_ins;
-                    }
-                }
-            }
-            wave_lds_fence();
+            n_dist += static_cast<unsigned long long>(b.hi_w);
+            scan_range<K>(spos, b.lo_w, b.hi_w, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
         } else {
             const Box c0 = nodes[2 * node];
             const Box c1 = nodes[2 * node + 1];
@@ -123,8 +154,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
             const bool need0 = wave_any(valid && d0 <= kth);
             const bool need1 = wave_any(valid && d1 <= kth);
             const int votes0 = __popcll(__ballot(valid && d0 <= d1));
-            const int nvalid = __popcll(__ballot(valid));
-            const bool near0 = 2 * votes0 >= nvalid;
+            const bool near0 = 2 * votes0 >= static_cast<int>(n_valid_lanes);
             const int first = near0 ? 2 * node + 1 : 2 * node;   // pushed first = visited last
             const int second = near0 ? 2 * node : 2 * node + 1;
             const bool need_first = near0 ? need1 : need0;
@@ -181,14 +211,15 @@ __global__ void dens_finalize_kernel(const double* __restrict__ d64, int64_t n, 
 }  // namespace
 
 hipError_t launch_knn_dens(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad,
-                           const int32_t* inv_perm, float* rk_sorted, double* dens64,
+                           int64_t nleaf, int leaf_size, const int32_t* inv_perm, float* rk_sorted, double* dens64,
                            int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int64_t waves = (n + kWave - 1) / kWave;
     const unsigned blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    knn_dens_kernel<kSlots><<<blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, inv_perm,
-                                                          rk_sorted, dens64, nbr_out_orig, stats);
+    const int init_leaves = leaf_size >= kWave ? 1 : kWave / leaf_size;
+    knn_dens_kernel<kSlots><<<blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves,
+                                                          inv_perm, rk_sorted, dens64, nbr_out_orig, stats);
     return hipGetLastError();
 }
 

@@ -145,7 +145,9 @@ def test_global_mesh_variant_matches_oracle(ctx):
     buf = np.concatenate([parts["x"], parts["v"]], axis=1)
     o = oracle_path(buf, L, [])
     gpu_path(ctx, parts, nl, L, bf, [])
-    for nc in (16, 10):
+    # nc not a multiple of the tile side, and tiles wider than the mesh
+    for nc, tile in ((16, 8), (10, 8), (7, 3), (13, 5), (5, 8)):
+        ctx.set_option(A.OPT_TILE, tile)
         ctx.assign_on_global_mesh(nc, -1, (0, 0, 0))
         gm = ctx.get_mesh(nc)
         om = O.assign_on_mesh(buf[:, :3], buf[:, 3:], o["dens32"], o["rk"], np.zeros(len(buf), np.int32), L, nc,
