@@ -99,6 +99,9 @@ def lib() -> C.CDLL:
         "asm_write_mesh_separate": (i32, [C.c_char_p, i32, vp, i32]),
         "asm_get_tree": (i32, [vp, vp, vp, P(i64), P(i64), P(i32)]),
         "asm_get_sorted_results": (i32, [vp, vp, vp]),
+        "asm_calibration_copy": (i32, [vp, vp, i64]),
+        "asm_device_alloc": (i32, [P(vp), i64]),
+        "asm_device_free": (i32, [vp]),
         "asm_timer_count": (i32, []),
         "asm_timer_name": (C.c_char_p, [i32]),
         "asm_get_timer": (i32, [vp, i32, P(f64), P(i64)]),

@@ -1,0 +1,52 @@
+"""Workload for hardware-counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE).
+
+Runs, in one process: a plain device copy of a known number of bytes (the
+calibration), then one pass of the smoothing path on the 256^3 Zel'dovich
+workload (tree, kNN + density, deposition of nc = 64, 128, 256).  Dividing
+the counter reading of each smoothing kernel by the copy kernel's reading per
+copied byte gives that kernel's HBM bytes per launch.
+
+    rocprofv3 --pmc FETCH_SIZE --kernel-trace -d <dir> -o fetch -- \\
+        python3 -m asmooth_amd.profile_traffic
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+
+import asmooth_amd as A
+from asmooth_amd import synth
+
+COPY_BYTES = 1 << 30
+
+
+def main() -> int:
+    n_side = int(sys.argv[1]) if len(sys.argv) > 1 else 256
+    L, bf = 2.0 * n_side, 0.05
+    ncs = [n_side // 4, n_side // 2, n_side]
+    lib = A.lib()
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if lib.asm_device_alloc(C.byref(p), COPY_BYTES) != 0:
+            raise A.AsmoothError("device allocation failed")
+    if lib.asm_calibration_copy(dst, src, COPY_BYTES) != 0:
+        raise A.AsmoothError("calibration copy failed")
+    print(f"calibration copy: {COPY_BYTES} bytes", flush=True)
+    xv = synth.zeldovich(n=n_side, L=L, rms=2.5, seed=1)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    with A.Context(0) as ctx:
+        ctx.set_particles(parts, nl, L)
+        ctx.construct(-bf * L, (1 + bf) * L)
+        ctx.neighbor_search(32)
+        for nc in ncs:
+            ctx.assign_on_mesh(nc, -1)
+            ctx.mesh_total(nc)
+        ctx.synchronize()
+    print(f"path: n_buf {len(parts)} n_local {nl} ncs {ncs}", flush=True)
+    lib.asm_device_free(src)
+    lib.asm_device_free(dst)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

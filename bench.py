@@ -67,7 +67,7 @@ def run_step(ctx, L: float, bf: float, ncs):
     return totals
 
 
-def cpu_baseline(xv: np.ndarray, L: float, bf: float, ncs, nn: int = 4):
+def cpu_baseline(xv: np.ndarray, L: float, bf: float, ncs, nn: int = 2):
     """The reference's construct / neighbour loop / assign_on_mesh (compiled from
     its sources, oracle/_ref) on one node share of an nn^3 decomposition of the
     same particle set, with meshes of the same cell size."""

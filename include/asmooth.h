@@ -192,6 +192,13 @@ int asm_reset_timers(asm_context* ctx);
  * evaluations, list insertions. */
 int asm_get_search_stats(asm_context* ctx, uint64_t out[4]);
 
+/* Measurement helpers (no counterpart in the reference): a plain device copy
+ * of `bytes` (multiple of 16) whose hardware byte counters calibrate those of
+ * the smoothing kernels, and raw device allocation for it. */
+int asm_calibration_copy(void* dst, const void* src, int64_t bytes);
+int asm_device_alloc(void** p, int64_t bytes);
+int asm_device_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif
