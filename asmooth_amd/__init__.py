@@ -260,7 +260,7 @@ class Context:
 
     def get_sorted_results(self):
         rk = np.empty(self.n, np.float32)
-        d64 = np.empty(self.n, np.float64)
+        d64 = np.empty(self.n, np.float32)
         self._check(self._L.asm_get_sorted_results(self._h, _ptr(rk), _ptr(d64)), "asm_get_sorted_results")
         return rk, d64
 

@@ -45,6 +45,7 @@ enum TimerId {
     T_SORT = 0,
     T_TREE,
     T_KNN,
+    T_DENS,
     T_DEPOSIT_LIST,
     T_DEPOSIT_TILES,
     T_REDUCE,
@@ -52,7 +53,7 @@ enum TimerId {
     T_HALO,
     T_COUNT
 };
-const char* kTimerNames[T_COUNT] = {"sort", "tree", "knn_dens", "deposit_lists", "deposit_tiles",
+const char* kTimerNames[T_COUNT] = {"sort", "tree", "knn", "dens_gather", "deposit_lists", "deposit_tiles",
                                     "mesh_reduce", "normalize", "mark_halo"};
 
 struct MeshDev {
@@ -87,8 +88,8 @@ struct asm_context {
     DevBuf<Box> nodes;
     int64_t nleaf = 0, nleaf_pad = 0;
     // search results (sorted order)
-    DevBuf<float> rk, dens;
-    DevBuf<double> dens64;
+    DevBuf<float> rk, dens, kmax;
+    DevBuf<uint64_t> kkey;
     DevBuf<int32_t> nbr;
     DevBuf<KnnStats> kstats;
     KnnStats last_stats{};
@@ -355,7 +356,8 @@ int asm_neighbor_search(asm_context* ctx, int k)
     hipStream_t s = ctx->stream;
     HIPC(ctx->rk.ensure(n));
     HIPC(ctx->dens.ensure(n));
-    HIPC(ctx->dens64.ensure(n));
+    HIPC(ctx->kkey.ensure(n));
+    HIPC(ctx->kmax.ensure(static_cast<size_t>(2 * ctx->nleaf_pad)));
     HIPC(ctx->kstats.ensure(1));
     int32_t* nbr = nullptr;
     if (ctx->keep_neighbors) {
@@ -364,12 +366,15 @@ int asm_neighbor_search(asm_context* ctx, int k)
     }
     {
         StageTimer t(ctx, T_KNN);
-        HIPC(launch_fill_zero_f64(ctx->dens64.p, n, s));
         HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->kstats.p), sizeof(KnnStats) / 4, s));
-        HIPC(launch_knn_dens(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->nleaf, ctx->leaf_size,
-                             ctx->inv_perm.p, ctx->rk.p,
-                             ctx->dens64.p, nbr, ctx->kstats.p, s));
-        HIPC(launch_dens_finalize(ctx->dens64.p, n, ctx->dens.p, s));
+        HIPC(launch_knn(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->nleaf, ctx->leaf_size, ctx->rk.p,
+                        ctx->kkey.p, nbr, ctx->kstats.p, s));
+    }
+    {
+        StageTimer t(ctx, T_DENS);
+        HIPC(launch_kmax(ctx->kkey.p, n, ctx->leaf_size, ctx->nleaf, ctx->nleaf_pad, ctx->kmax.p, s));
+        HIPC(launch_dens_gather(ctx->spos.p, ctx->kkey.p, ctx->rk.p, n, ctx->nodes.p, ctx->kmax.p, ctx->nleaf_pad,
+                                ctx->dens.p, s));
     }
     HIPC(hipMemcpyAsync(&ctx->last_stats, ctx->kstats.p, sizeof(KnnStats), hipMemcpyDeviceToHost, s));
     ctx->have_search = true;
@@ -640,15 +645,14 @@ int asm_get_tree(asm_context* ctx, float* spos, int32_t* nodes, int64_t* nleaf, 
     return ASM_OK;
 }
 
-int asm_get_sorted_results(asm_context* ctx, float* rk, double* dens64)
+int asm_get_sorted_results(asm_context* ctx, float* rk, float* dens)
 {
     if (!ctx) return ASM_ERR_ARG;
     if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "no search results");
     HIPC(hipSetDevice(ctx->device));
     HIPC(hipStreamSynchronize(ctx->stream));
     if (rk) HIPC(hipMemcpy(rk, ctx->rk.p, sizeof(float) * static_cast<size_t>(ctx->n), hipMemcpyDeviceToHost));
-    if (dens64)
-        HIPC(hipMemcpy(dens64, ctx->dens64.p, sizeof(double) * static_cast<size_t>(ctx->n), hipMemcpyDeviceToHost));
+    if (dens) HIPC(hipMemcpy(dens, ctx->dens.p, sizeof(float) * static_cast<size_t>(ctx->n), hipMemcpyDeviceToHost));
     return ASM_OK;
 }
 

@@ -42,11 +42,16 @@ struct KnnStats {
     unsigned long long dist_evals;
     unsigned long long inserts;
 };
-hipError_t launch_knn_dens(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad,
-                           int64_t nleaf, int leaf_size, const int32_t* inv_perm, float* rk_sorted, double* dens64,
-                           int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s);
-// dens (sorted order, float) from the FP64 sums.
-hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s);
+// kNN: rk = sqrtf(d2 of the 33rd key) and that key, per sorted particle.
+hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
+                      int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, int32_t* nbr_out_orig, KnnStats* stats,
+                      hipStream_t s);
+// Per-node maximum of the 33rd squared distance (2*nleaf_pad floats).
+hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t nleaf, int64_t nleaf_pad, float* kmax,
+                       hipStream_t s);
+// Density by gather (sorted order, float).
+hipError_t launch_dens_gather(const float4* spos, const uint64_t* kkey, const float* rk, int64_t n, const Box* nodes,
+                              const float* kmax, int64_t nleaf_pad, float* dens, hipStream_t s);
 
 // ---- deposit.hip -------------------------------------------------------------
 struct DepositArgs {

@@ -16,9 +16,14 @@
 // keys (FP32 squared distance bits, original index) in registers.  That is
 // the canonical (d2, index) order: ties are broken by index, and distances
 // use exactly the reference's FP32 arithmetic, so h and the neighbour sets
-// are reproducible bit for bit.  h = sqrtf(d2 of slot 32) (nbr_search.cpp:73),
-// the density kernel goes to slots 0..31 (nbr_search.cpp:78-84: the particle
-// itself and its 31 nearest others), accumulated in FP64 with native atomics.
+// are reproducible bit for bit.  h = sqrtf(d2 of slot 32) (nbr_search.cpp:73).
+//
+// The density scatter of nbr_search.cpp:78-84 (particle p adds its tent
+// kernel to its 32 nearest, itself included) is computed as a gather: q
+// receives W(p, q, h_p) from every p whose 33rd key is above (d2(p,q), q).
+// The kNN kernel stores each particle's 33rd key; a second walk, pruned by
+// the largest 33rd distance below each node, sums the terms in FP64
+// registers - no atomics, and a result independent of scheduling.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -93,8 +98,8 @@ template <int K>
 __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad,
     int64_t nleaf, int leaf_size, int init_leaves,
-    const int32_t* __restrict__ inv_perm, float* __restrict__ rk_sorted, double* __restrict__ dens64,
-    int32_t* __restrict__ nbr_out, KnnStats* __restrict__ stats)
+    float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted, int32_t* __restrict__ nbr_out,
+    KnnStats* __restrict__ stats)
 {
     __shared__ float4 stage[kWavesPerBlock][kWave];
     const int lane = threadIdx.x % kWave;
@@ -172,17 +177,8 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
 
     if (valid) {
         const float kd = key_d2(key[K - 1]);
-        const float h = sqrtf(kd);
-        rk_sorted[q] = h;
-        const float hinv = 1.0f / h;
-#pragma unroll
-        for (int s = 0; s < K - 1; ++s) {
-            const uint32_t o = key_idx(key[s]);
-            const int32_t sq = inv_perm[o];
-            const float4 c = spos[sq];
-            const float w = tent_weight(qx, qy, qz, c.x, c.y, c.z, hinv);
-            unsafeAtomicAdd(&dens64[sq], static_cast<double>(w));
-        }
+        rk_sorted[q] = sqrtf(kd);
+        kkey_sorted[q] = key[K - 1];
         if (nbr_out) {
             const int64_t oq = static_cast<int64_t>(__float_as_uint(me.w));
 #pragma unroll
@@ -202,32 +198,126 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     }
 }
 
-__global__ void dens_finalize_kernel(const double* __restrict__ d64, int64_t n, float* __restrict__ d)
+
+// Largest 33rd squared distance over the particles below each node.
+__global__ void kmax_leaves_kernel(const uint64_t* __restrict__ kkey, int64_t n, int leaf_size, int64_t nleaf,
+                                   int64_t nleaf_pad, float* __restrict__ kmax)
 {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = static_cast<float>(d64[i]);
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nleaf_pad) return;
+    float m = -INFINITY;
+    if (j < nleaf) {
+        const int64_t start = j * leaf_size;
+        const int64_t end = (start + leaf_size < n) ? start + leaf_size : n;
+        for (int64_t i = start; i < end; ++i) m = fmaxf(m, key_d2(kkey[i]));
+    }
+    kmax[nleaf_pad + j] = m;
+}
+
+__global__ void kmax_level_kernel(float* __restrict__ kmax, int64_t first, int64_t count)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t i = first + t;
+    kmax[i] = fmaxf(kmax[2 * i], kmax[2 * i + 1]);
+}
+
+// dens[q] = sum over p with (d2(p,q), q) < kkey[p] of W(p, q, 1/h_p), in FP64.
+// d2 is symmetric bit for bit (fl(a-b) = -fl(b-a)), so the test reproduces
+// "q is in p's 32 nearest" exactly; W uses p as x and q as y as
+// nbr_search.cpp:81 does.  One query per lane, packet walk as the search.
+__global__ __launch_bounds__(kKnnThreads) void dens_gather_kernel(
+    const float4* __restrict__ spos, const uint64_t* __restrict__ kkey, const float* __restrict__ rk, int64_t n,
+    const Box* __restrict__ nodes, const float* __restrict__ kmax, int64_t nleaf_pad, float* __restrict__ dens)
+{
+    __shared__ float4 stage_p[kWavesPerBlock][kWave];
+    __shared__ float4 stage_k[kWavesPerBlock][kWave];   // (kkey lo, kkey hi, hinv, -)
+    const int lane = threadIdx.x % kWave;
+    const int wid = threadIdx.x / kWave;
+    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid) * kWave;
+    if (wave0 >= n) return;
+    const int64_t q = wave0 + lane;
+    const bool valid = q < n;
+    const float4 me = spos[valid ? q : n - 1];
+    const float qx = me.x, qy = me.y, qz = me.z;
+    const uint32_t qidx = __float_as_uint(me.w);
+    double acc = 0.0;
+
+    int stack = (lane == 0) ? 1 : 0;
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int node = __builtin_amdgcn_readlane(stack, sp);
+        const Box b = nodes[node];
+        const float km = kmax[node];
+        if (!wave_any(valid && box_d2(b, qx, qy, qz) <= km)) continue;
+        if (node >= nleaf_pad) {
+            const int start = b.lo_w;
+            const int cnt = b.hi_w;
+            if (lane < cnt) {
+                const int64_t i = start + lane;
+                stage_p[wid][lane] = spos[i];
+                const uint64_t kk = kkey[i];
+                stage_k[wid][lane] = make_float4(__uint_as_float(static_cast<uint32_t>(kk)),
+                                                 __uint_as_float(static_cast<uint32_t>(kk >> 32)),
+                                                 1.0f / rk[i], 0.0f);
+            }
+            wave_lds_fence();
+            for (int j = 0; j < cnt; ++j) {
+                const float4 p = stage_p[wid][j];
+                const float4 pk = stage_k[wid][j];
+                const float d2 = norm2(p.x, p.y, p.z, qx, qy, qz);
+                const uint64_t kp = (static_cast<uint64_t>(__float_as_uint(pk.y)) << 32) | __float_as_uint(pk.x);
+                if (make_key(d2, qidx) < kp)
+                    acc += static_cast<double>(tent_weight(p.x, p.y, p.z, qx, qy, qz, pk.z));
+            }
+            wave_lds_fence();
+        } else {
+            stack = (lane == sp) ? 2 * node + 1 : stack;
+            ++sp;
+            stack = (lane == sp) ? 2 * node : stack;
+            ++sp;
+        }
+    }
+    if (valid) dens[q] = static_cast<float>(acc);
 }
 
 }  // namespace
 
-hipError_t launch_knn_dens(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad,
-                           int64_t nleaf, int leaf_size, const int32_t* inv_perm, float* rk_sorted, double* dens64,
-                           int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s)
+hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
+                      int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, int32_t* nbr_out_orig, KnnStats* stats,
+                      hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int64_t waves = (n + kWave - 1) / kWave;
     const unsigned blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     const int init_leaves = leaf_size >= kWave ? 1 : kWave / leaf_size;
     knn_dens_kernel<kSlots><<<blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves,
-                                                          inv_perm, rk_sorted, dens64, nbr_out_orig, stats);
+                                                          rk_sorted, kkey_sorted, nbr_out_orig, stats);
     return hipGetLastError();
 }
 
-hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s)
+hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t nleaf, int64_t nleaf_pad, float* kmax,
+                       hipStream_t s)
+{
+    const int th = 256;
+    kmax_leaves_kernel<<<static_cast<unsigned>((nleaf_pad + th - 1) / th), th, 0, s>>>(kkey, n, leaf_size, nleaf,
+                                                                                      nleaf_pad, kmax);
+    ASM_HIP_CHECK(hipGetLastError());
+    for (int64_t first = nleaf_pad / 2; first >= 1; first /= 2) {
+        kmax_level_kernel<<<static_cast<unsigned>((first + th - 1) / th), th, 0, s>>>(kmax, first, first);
+        ASM_HIP_CHECK(hipGetLastError());
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_dens_gather(const float4* spos, const uint64_t* kkey, const float* rk, int64_t n, const Box* nodes,
+                              const float* kmax, int64_t nleaf_pad, float* dens, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const int th = 256;
-    dens_finalize_kernel<<<static_cast<unsigned>((n + th - 1) / th), th, 0, s>>>(dens64, n, dens);
+    const int64_t waves = (n + kWave - 1) / kWave;
+    const unsigned blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    dens_gather_kernel<<<blocks, kKnnThreads, 0, s>>>(spos, kkey, rk, n, nodes, kmax, nleaf_pad, dens);
     return hipGetLastError();
 }
 

@@ -143,7 +143,7 @@ def main(argv=None) -> int:
 
     # Roofline of the dominant kernel (largest total time in the timed steps).
     alg = {
-        "knn_dens": (BYTES_KNN * nbuf, "kNN + h + FP64 density scatter; 20 B/particle x N_buf"),
+        "knn": (BYTES_KNN * nbuf, "kNN + h (+ 33rd key for the density gather); 20 B/particle x N_buf"),
         "deposit_tiles": (sum(BYTES_DEPOSIT * nbuf + BYTES_CELL * nc ** 3 for nc in ncs) / len(ncs),
                           "tile deposition; 36 B/particle x N_buf + 24 B/cell x nc^3, mean over the nc list"),
         "sort": (BYTES_TREE * nbuf, "Morton radix sort; 24 B/particle"),

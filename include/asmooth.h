@@ -142,8 +142,8 @@ int asm_get_neighbors(asm_context* ctx, int32_t* nbr);
  * node i has children 2i and 2i+1, leaves are nleaf_pad .. 2*nleaf_pad-1). */
 int asm_get_tree(asm_context* ctx, float* spos, int32_t* nodes, int64_t* nleaf, int64_t* nleaf_pad,
                  int32_t* leaf_size);
-/* Search results in sorted (tree) order: rk and the FP64 density sums. */
-int asm_get_sorted_results(asm_context* ctx, float* rk, double* dens64);
+/* Search results in sorted (tree) order: rk and dens. */
+int asm_get_sorted_results(asm_context* ctx, float* rk, float* dens);
 
 /* Host steps either side of the path. */
 /* Buffered set of a single periodic rank: the 26 periodic images of
