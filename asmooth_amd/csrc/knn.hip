@@ -67,12 +67,45 @@ __device__ __forceinline__ void wave_lds_fence()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+constexpr int kBufSlots = 16;  // per-lane candidate buffer (LDS) between merges
+
+// Candidate buffer of one wavefront: slot-major, lane-minor, so the 64
+// lanes' appends of one slot hit 64 consecutive 8-byte words.
+struct WaveBuf {
+    uint64_t k[kBufSlots][kWave];
+};
+
+// Merge every lane's buffered candidates into its sorted list.  The loop
+// runs to the fullest lane's count; a lane inserts only entries still below
+// its current 33rd key.  Buffering lets one pass of the 33-slot insertion
+// serve all lanes that accepted candidates since the last merge, instead of
+// one pass per candidate that any lane accepted.
+template <int K>
+__device__ __forceinline__ void merge_buffer(WaveBuf* buf, int lane, int& cnt, uint64_t (&key)[K], float& kth,
+                                             unsigned long long& n_ins)
+{
+    int m = cnt;
+    for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    for (int j = 0; j < m; ++j) {
+        if (j < cnt) {
+            const uint64_t ck = buf->k[j][lane];
+            if (ck < key[K - 1]) {
+                sorted_insert<K>(key, ck);
+                ++n_ins;
+            }
+        }
+    }
+    kth = key_d2(key[K - 1]);
+    cnt = 0;
+}
+
 // Every lane tests particles [start, start + cnt) of the sorted array,
-// staged through LDS 64 at a time.
+// staged through LDS 64 at a time; candidates below the lane's current 33rd
+// key go to its buffer, and the buffers are merged when one fills.
 template <int K>
 __device__ __forceinline__ void scan_range(const float4* __restrict__ spos, int64_t start, int64_t cnt,
-                                           float4* stage, int lane, float qx, float qy, float qz,
-                                           uint64_t (&key)[K], float& kth, unsigned long long& n_ins)
+                                           float4* stage, WaveBuf* buf, int lane, float qx, float qy, float qz,
+                                           uint64_t (&key)[K], float& kth, int& nbuf, unsigned long long& n_ins)
 {
     for (int64_t base = 0; base < cnt; base += kWave) {
         const int m = static_cast<int>(cnt - base < kWave ? cnt - base : kWave);
@@ -81,14 +114,10 @@ __device__ __forceinline__ void scan_range(const float4* __restrict__ spos, int6
         for (int j = 0; j < m; ++j) {
             const float4 c = stage[j];
             const float d2 = norm2(qx, qy, qz, c.x, c.y, c.z);
-            if (d2 <= kth) {
-                const uint64_t ck = make_key(d2, __float_as_uint(c.w));
-                if (ck < key[K - 1]) {
-                    sorted_insert<K>(key, ck);
-                    kth = key_d2(key[K - 1]);
-                    ++n_ins;
-                }
-            }
+            const uint64_t ck = make_key(d2, __float_as_uint(c.w));
+            const bool acc = (d2 <= kth) && (ck < key[K - 1]);
+            if (acc) buf->k[nbuf++][lane] = ck;
+            if (__ballot(nbuf == kBufSlots)) merge_buffer<K>(buf, lane, nbuf, key, kth, n_ins);
         }
         wave_lds_fence();
     }
@@ -102,6 +131,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     KnnStats* __restrict__ stats)
 {
     __shared__ float4 stage[kWavesPerBlock][kWave];
+    __shared__ WaveBuf bufs[kWavesPerBlock];
     const int lane = threadIdx.x % kWave;
     const int wid = threadIdx.x / kWave;
     const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid) * kWave;
@@ -110,6 +140,8 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     const bool valid = q < n;
     const float4 me = spos[valid ? q : n - 1];
     const float qx = me.x, qy = me.y, qz = me.z;
+    WaveBuf* buf = &bufs[wid];
+    int nbuf = 0;
 
     uint64_t key[K];
 #pragma unroll
@@ -131,7 +163,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     {
         const int64_t p0 = l0 * leaf_size;
         const int64_t p1 = ((l1 + 1) * leaf_size < n) ? (l1 + 1) * leaf_size : n;
-        scan_range<K>(spos, p0, p1 - p0, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
+        scan_range<K>(spos, p0, p1 - p0, stage[wid], buf, lane, qx, qy, qz, key, kth, nbuf, n_ins);
         n_leaf += static_cast<unsigned long long>(l1 - l0 + 1);
         n_dist += static_cast<unsigned long long>(p1 - p0);
     }
@@ -150,7 +182,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
             if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
             ++n_leaf;
             n_dist += static_cast<unsigned long long>(b.hi_w);
-            scan_range<K>(spos, b.lo_w, b.hi_w, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
+            scan_range<K>(spos, b.lo_w, b.hi_w, stage[wid], buf, lane, qx, qy, qz, key, kth, nbuf, n_ins);
         } else {
             const Box c0 = nodes[2 * node];
             const Box c1 = nodes[2 * node + 1];
@@ -174,6 +206,8 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
             }
         }
     }
+
+    merge_buffer<K>(buf, lane, nbuf, key, kth, n_ins);
 
     if (valid) {
         const float kd = key_d2(key[K - 1]);
