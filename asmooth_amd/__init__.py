@@ -45,6 +45,7 @@ OPT_KEEP_NEIGHBORS = 1
 OPT_PROFILE = 2
 OPT_LEAF_SIZE = 3
 OPT_TILE = 4
+OPT_DETERMINISTIC = 5
 
 
 class AsmoothError(RuntimeError):

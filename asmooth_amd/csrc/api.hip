@@ -53,7 +53,7 @@ enum TimerId {
     T_HALO,
     T_COUNT
 };
-const char* kTimerNames[T_COUNT] = {"sort", "tree", "knn", "dens_gather", "deposit_lists", "deposit_tiles",
+const char* kTimerNames[T_COUNT] = {"sort", "tree", "knn", "dens", "deposit_lists", "deposit_tiles",
                                     "mesh_reduce", "normalize", "mark_halo"};
 
 struct MeshDev {
@@ -90,6 +90,8 @@ struct asm_context {
     // search results (sorted order)
     DevBuf<float> rk, dens, kmax;
     DevBuf<uint64_t> kkey;
+    DevBuf<double> dens64;
+    int deterministic = 0;
     DevBuf<int32_t> nbr;
     DevBuf<KnnStats> kstats;
     KnnStats last_stats{};
@@ -264,6 +266,7 @@ int asm_set_option(asm_context* ctx, int option, int value)
         ctx->have_tree = false;
         ctx->have_search = false;
         return ASM_OK;
+    case ASM_OPT_DETERMINISTIC: ctx->deterministic = value ? 1 : 0; return ASM_OK;
     case ASM_OPT_TILE:
         if (value < 1 || value > 8) return fail(ctx, ASM_ERR_ARG, "tile %d outside 1..8", value);
         ctx->tile = value;
@@ -364,17 +367,24 @@ int asm_neighbor_search(asm_context* ctx, int k)
         HIPC(ctx->nbr.ensure(static_cast<size_t>(n) * kSlots));
         nbr = ctx->nbr.p;
     }
+    if (!ctx->deterministic) HIPC(ctx->dens64.ensure(n));
     {
         StageTimer t(ctx, T_KNN);
         HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->kstats.p), sizeof(KnnStats) / 4, s));
+        if (!ctx->deterministic) HIPC(launch_fill_zero_f64(ctx->dens64.p, n, s));
         HIPC(launch_knn(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->nleaf, ctx->leaf_size, ctx->rk.p,
-                        ctx->kkey.p, nbr, ctx->kstats.p, s));
+                        ctx->kkey.p, ctx->inv_perm.p, ctx->deterministic ? nullptr : ctx->dens64.p, nbr, ctx->kstats.p,
+                        s));
     }
     {
         StageTimer t(ctx, T_DENS);
-        HIPC(launch_kmax(ctx->kkey.p, n, ctx->leaf_size, ctx->nleaf, ctx->nleaf_pad, ctx->kmax.p, s));
-        HIPC(launch_dens_gather(ctx->spos.p, ctx->kkey.p, ctx->rk.p, n, ctx->nodes.p, ctx->kmax.p, ctx->nleaf_pad,
-                                ctx->dens.p, s));
+        if (ctx->deterministic) {
+            HIPC(launch_kmax(ctx->kkey.p, n, ctx->leaf_size, ctx->nleaf, ctx->nleaf_pad, ctx->kmax.p, s));
+            HIPC(launch_dens_gather(ctx->spos.p, ctx->kkey.p, ctx->rk.p, n, ctx->nodes.p, ctx->kmax.p,
+                                    ctx->nleaf_pad, ctx->dens.p, s));
+        } else {
+            HIPC(launch_dens_finalize(ctx->dens64.p, n, ctx->dens.p, s));
+        }
     }
     HIPC(hipMemcpyAsync(&ctx->last_stats, ctx->kstats.p, sizeof(KnnStats), hipMemcpyDeviceToHost, s));
     ctx->have_search = true;

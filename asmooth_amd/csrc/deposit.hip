@@ -242,8 +242,15 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
     const float dxc = a.boxsize / static_cast<float>(a.nc);
     const uint32_t begin = tile_start[tile];
     const uint32_t count = tile_cnt[tile];
-    for (uint32_t li = tid; li < count; li += kDepThreads) {
-        const int64_t i = list[begin + li];
+    // Lanes of a wavefront take list entries `stride` apart: the list is in
+    // Morton order, so neighbouring entries share cells, and LDS atomics of
+    // one instruction to the same address would serialise.
+    const uint32_t stride = (count + kWave - 1) / kWave;
+    const uint32_t span = stride * kWave;
+    for (uint32_t li = tid; li < span; li += kDepThreads) {
+        const uint32_t e = (li % kWave) * stride + li / kWave;
+        if (e >= count) continue;
+        const int64_t i = list[begin + e];
         const PartRange r = part_range(a, i);
         const float hinv = 1.0f / r.h;
         const float ds = dxc * hinv;

@@ -43,9 +43,11 @@ struct KnnStats {
     unsigned long long inserts;
 };
 // kNN: rk = sqrtf(d2 of the 33rd key) and that key, per sorted particle.
+// With dens64 non-null the kernel also scatters the density terms (FP64 atomics).
 hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
-                      int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, int32_t* nbr_out_orig, KnnStats* stats,
-                      hipStream_t s);
+                      int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, const int32_t* inv_perm, double* dens64,
+                      int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s);
+hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s);
 // Per-node maximum of the 33rd squared distance (2*nleaf_pad floats).
 hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t nleaf, int64_t nleaf_pad, float* kmax,
                        hipStream_t s);

@@ -19,11 +19,13 @@
 // are reproducible bit for bit.  h = sqrtf(d2 of slot 32) (nbr_search.cpp:73).
 //
 // The density scatter of nbr_search.cpp:78-84 (particle p adds its tent
-// kernel to its 32 nearest, itself included) is computed as a gather: q
-// receives W(p, q, h_p) from every p whose 33rd key is above (d2(p,q), q).
-// The kNN kernel stores each particle's 33rd key; a second walk, pruned by
-// the largest 33rd distance below each node, sums the terms in FP64
-// registers - no atomics, and a result independent of scheduling.
+// kernel to its 32 nearest, itself included) runs at the end of the search
+// with native FP64 atomics (the FP32 result is the rounding of an FP64 sum,
+// so the summation order does not show).  A bitwise-reproducible mode
+// computes it instead as a gather: q receives W(p, q, h_p) from every p whose
+// 33rd key is above (d2(p,q), q); the kNN kernel stores each 33rd key and a
+// second walk, pruned by the largest 33rd distance below each node, sums the
+// terms in FP64 registers in a fixed order.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -67,45 +69,15 @@ __device__ __forceinline__ void wave_lds_fence()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kBufSlots = 16;  // per-lane candidate buffer (LDS) between merges
-
-// Candidate buffer of one wavefront: slot-major, lane-minor, so the 64
-// lanes' appends of one slot hit 64 consecutive 8-byte words.
-struct WaveBuf {
-    uint64_t k[kBufSlots][kWave];
-};
-
-// Merge every lane's buffered candidates into its sorted list.  The loop
-// runs to the fullest lane's count; a lane inserts only entries still below
-// its current 33rd key.  Buffering lets one pass of the 33-slot insertion
-// serve all lanes that accepted candidates since the last merge, instead of
-// one pass per candidate that any lane accepted.
-template <int K>
-__device__ __forceinline__ void merge_buffer(WaveBuf* buf, int lane, int& cnt, uint64_t (&key)[K], float& kth,
-                                             unsigned long long& n_ins)
-{
-    int m = cnt;
-    for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
-    for (int j = 0; j < m; ++j) {
-        if (j < cnt) {
-            const uint64_t ck = buf->k[j][lane];
-            if (ck < key[K - 1]) {
-                sorted_insert<K>(key, ck);
-                ++n_ins;
-            }
-        }
-    }
-    kth = key_d2(key[K - 1]);
-    cnt = 0;
-}
-
 // Every lane tests particles [start, start + cnt) of the sorted array,
-// staged through LDS 64 at a time; candidates below the lane's current 33rd
-// key go to its buffer, and the buffers are merged when one fills.
+// staged through LDS 64 at a time.  When any lane accepts a candidate, the
+// whole wavefront runs the insertion with uniform control flow; lanes that
+// did not accept insert an empty key, which leaves their array unchanged.
+// (Divergent insertion makes the compiler keep two copies of the 33 keys.)
 template <int K>
 __device__ __forceinline__ void scan_range(const float4* __restrict__ spos, int64_t start, int64_t cnt,
-                                           float4* stage, WaveBuf* buf, int lane, float qx, float qy, float qz,
-                                           uint64_t (&key)[K], float& kth, int& nbuf, unsigned long long& n_ins)
+                                           float4* stage, int lane, float qx, float qy, float qz,
+                                           uint64_t (&key)[K], float& kth, unsigned long long& n_ins)
 {
     for (int64_t base = 0; base < cnt; base += kWave) {
         const int m = static_cast<int>(cnt - base < kWave ? cnt - base : kWave);
@@ -114,10 +86,13 @@ __device__ __forceinline__ void scan_range(const float4* __restrict__ spos, int6
         for (int j = 0; j < m; ++j) {
             const float4 c = stage[j];
             const float d2 = norm2(qx, qy, qz, c.x, c.y, c.z);
-            const uint64_t ck = make_key(d2, __float_as_uint(c.w));
-            const bool acc = (d2 <= kth) && (ck < key[K - 1]);
-            if (acc) buf->k[nbuf++][lane] = ck;
-            if (__ballot(nbuf == kBufSlots)) merge_buffer<K>(buf, lane, nbuf, key, kth, n_ins);
+            const bool acc = d2 <= kth;
+            if (__ballot(acc)) {
+                const uint64_t ck = acc ? make_key(d2, __float_as_uint(c.w)) : kEmptyKey;
+                sorted_insert<K>(key, ck);
+                kth = key_d2(key[K - 1]);
+                n_ins += acc ? 1u : 0u;
+            }
         }
         wave_lds_fence();
     }
@@ -127,11 +102,10 @@ template <int K>
 __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad,
     int64_t nleaf, int leaf_size, int init_leaves,
-    float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted, int32_t* __restrict__ nbr_out,
-    KnnStats* __restrict__ stats)
+    float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted, const int32_t* __restrict__ inv_perm,
+    double* __restrict__ dens64, int32_t* __restrict__ nbr_out, KnnStats* __restrict__ stats)
 {
     __shared__ float4 stage[kWavesPerBlock][kWave];
-    __shared__ WaveBuf bufs[kWavesPerBlock];
     const int lane = threadIdx.x % kWave;
     const int wid = threadIdx.x / kWave;
     const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid) * kWave;
@@ -140,8 +114,6 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     const bool valid = q < n;
     const float4 me = spos[valid ? q : n - 1];
     const float qx = me.x, qy = me.y, qz = me.z;
-    WaveBuf* buf = &bufs[wid];
-    int nbuf = 0;
 
     uint64_t key[K];
 #pragma unroll
@@ -163,7 +135,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     {
         const int64_t p0 = l0 * leaf_size;
         const int64_t p1 = ((l1 + 1) * leaf_size < n) ? (l1 + 1) * leaf_size : n;
-        scan_range<K>(spos, p0, p1 - p0, stage[wid], buf, lane, qx, qy, qz, key, kth, nbuf, n_ins);
+        scan_range<K>(spos, p0, p1 - p0, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
         n_leaf += static_cast<unsigned long long>(l1 - l0 + 1);
         n_dist += static_cast<unsigned long long>(p1 - p0);
     }
@@ -182,7 +154,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
             if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
             ++n_leaf;
             n_dist += static_cast<unsigned long long>(b.hi_w);
-            scan_range<K>(spos, b.lo_w, b.hi_w, stage[wid], buf, lane, qx, qy, qz, key, kth, nbuf, n_ins);
+            scan_range<K>(spos, b.lo_w, b.hi_w, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
         } else {
             const Box c0 = nodes[2 * node];
             const Box c1 = nodes[2 * node + 1];
@@ -207,12 +179,22 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
         }
     }
 
-    merge_buffer<K>(buf, lane, nbuf, key, kth, n_ins);
-
     if (valid) {
         const float kd = key_d2(key[K - 1]);
-        rk_sorted[q] = sqrtf(kd);
+        const float h = sqrtf(kd);
+        rk_sorted[q] = h;
         kkey_sorted[q] = key[K - 1];
+        if (dens64) {
+            // nbr_search.cpp:75-84: slots 0..K-2 (the particle itself and its
+            // 31 nearest others) receive W(p, q, 1/h_p); FP64 atomic sums.
+            const float hinv = 1.0f / h;
+#pragma unroll
+            for (int s = 0; s < K - 1; ++s) {
+                const int32_t sq = inv_perm[key_idx(key[s])];
+                const float4 c = spos[sq];
+                unsafeAtomicAdd(&dens64[sq], static_cast<double>(tent_weight(qx, qy, qz, c.x, c.y, c.z, hinv)));
+            }
+        }
         if (nbr_out) {
             const int64_t oq = static_cast<int64_t>(__float_as_uint(me.w));
 #pragma unroll
@@ -316,18 +298,32 @@ __global__ __launch_bounds__(kKnnThreads) void dens_gather_kernel(
     if (valid) dens[q] = static_cast<float>(acc);
 }
 
+__global__ void dens_finalize_kernel(const double* __restrict__ d64, int64_t n, float* __restrict__ d)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = static_cast<float>(d64[i]);
+}
+
 }  // namespace
 
+hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    dens_finalize_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(dens64, n, dens);
+    return hipGetLastError();
+}
+
 hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
-                      int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, int32_t* nbr_out_orig, KnnStats* stats,
-                      hipStream_t s)
+                      int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, const int32_t* inv_perm, double* dens64,
+                      int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int64_t waves = (n + kWave - 1) / kWave;
     const unsigned blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     const int init_leaves = leaf_size >= kWave ? 1 : kWave / leaf_size;
     knn_dens_kernel<kSlots><<<blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves,
-                                                          rk_sorted, kkey_sorted, nbr_out_orig, stats);
+                                                          rk_sorted, kkey_sorted, inv_perm, dens64, nbr_out_orig,
+                                                          stats);
     return hipGetLastError();
 }
 

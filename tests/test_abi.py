@@ -40,7 +40,7 @@ def test_record_layouts_match_reference_structs():
 def test_version_and_timer_names():
     assert b"gfx950" in A.lib().asm_version()
     names = A.timer_names()
-    assert "knn" in names and "dens_gather" in names and "deposit_tiles" in names
+    assert "knn" in names and "dens" in names and "deposit_tiles" in names
 
 
 def test_xv_round_trip(tmp_path):

@@ -204,3 +204,22 @@ def test_tree_invariants(ctx, kind, n_side, leaf):
         assert nodes[i, 7] == nodes[2 * i, 7] + nodes[2 * i + 1, 7]
     assert nodes[1, 7] == n
     ctx.set_option(A.OPT_LEAF_SIZE, 32)
+
+
+def test_deterministic_density_mode(ctx):
+    """ASM_OPT_DETERMINISTIC: density by an ordered FP64 gather, identical bits
+    run to run, same values as the oracle."""
+    xv, L, bf = small_set("blobs", 16, 21)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    pos = np.ascontiguousarray(parts["x"])
+    onbr, _, ork = O.knn(pos)
+    od = O.density(pos, onbr, ork)
+    ctx.set_option(A.OPT_DETERMINISTIC, 1)
+    try:
+        runs = [gpu_path(ctx, parts, nl, L, bf, [16]) for _ in range(2)]
+    finally:
+        ctx.set_option(A.OPT_DETERMINISTIC, 0)
+    assert np.array_equal(runs[0]["dens"].view(np.int32), runs[1]["dens"].view(np.int32))
+    assert np.array_equal(runs[0]["meshes"][16].view(np.int32), runs[1]["meshes"][16].view(np.int32))
+    assert np.array_equal(runs[0]["rk"], ork)
+    assert rel_err(runs[0]["dens"], od) <= DENS_RTOL
