@@ -20,7 +20,52 @@ from asmooth_amd import synth
 COPY_BYTES = 1 << 30
 
 
+def _short(name: str) -> str:
+    """Kernel name without namespaces and argument list."""
+    base = name.replace("(anonymous namespace)::", "").split("(")[0]
+    return base.split("::")[-1].replace("void ", "").strip()
+
+
+def summarize(stats_csv: str, fetch_csv: str, write_csv: str, command: str) -> str:
+    """Markdown summary: per-kernel time from the kernel-trace stats run and
+    HBM bytes per dispatch from the FETCH_SIZE / WRITE_SIZE runs, calibrated
+    by the copy kernel (bytes copied / counter reading, per counter)."""
+    import csv
+    import collections
+
+    def counters(fn, cname):
+        d = collections.defaultdict(list)
+        for r in csv.DictReader(open(fn)):
+            if r["Counter_Name"] == cname:
+                d[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]) * 1024.0)
+        return d
+
+    fetch = counters(fetch_csv, "FETCH_SIZE")
+    write = counters(write_csv, "WRITE_SIZE")
+    f_cal = COPY_BYTES / fetch["copy_kernel"][0]
+    w_cal = COPY_BYTES / write["copy_kernel"][0]
+    rows = list(csv.DictReader(open(stats_csv)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = [f"Command: `{command}`", "",
+           f"Calibration (copy kernel, {COPY_BYTES} bytes in and out): FETCH_SIZE x {f_cal:.3f}, "
+           f"WRITE_SIZE x {w_cal:.3f}.", "",
+           "| kernel | calls | avg ms | total % | HBM read MB/launch | HBM write MB/launch |",
+           "|---|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        k = _short(r["Name"])
+        fr = fetch.get(k)
+        wr = write.get(k)
+        rd = f"{sum(fr) / len(fr) * f_cal / 1e6:.1f}" if fr else "-"
+        wt = f"{sum(wr) / len(wr) * w_cal / 1e6:.1f}" if wr else "-"
+        out.append(f"| {k} | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | {float(r['Percentage']):.1f} | "
+                   f"{rd} | {wt} |")
+    return "\n".join(out) + "\n"
+
+
 def main() -> int:
+    if len(sys.argv) > 1 and sys.argv[1] == "summarize":
+        print(summarize(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5]))
+        return 0
     n_side = int(sys.argv[1]) if len(sys.argv) > 1 else 256
     L, bf = 2.0 * n_side, 0.05
     ncs = [n_side // 4, n_side // 2, n_side]
