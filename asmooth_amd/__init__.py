@@ -46,6 +46,7 @@ OPT_PROFILE = 2
 OPT_LEAF_SIZE = 3
 OPT_TILE = 4
 OPT_DETERMINISTIC = 5
+OPT_KNN_METHOD = 6
 
 
 class AsmoothError(RuntimeError):

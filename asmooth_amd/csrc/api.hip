@@ -92,9 +92,12 @@ struct asm_context {
     DevBuf<uint64_t> kkey;
     DevBuf<double> dens64;
     int deterministic = 0;
+    int knn_method = kKnnBoundCollect;
+    DevBuf<uint32_t> knn_flags, knn_leaf_n, knn_leaves;   // per wavefront of 64 sorted particles
+    DevBuf<float> knn_tau;                                // per sorted particle
+    DevBuf<double> dens_pull;
     DevBuf<int32_t> nbr;
-    DevBuf<KnnStats> kstats;
-    KnnStats last_stats{};
+    DevBuf<KnnStats> kstats;   // kStatSlots copies
     // deposition
     DevBuf<uint32_t> tile_cnt, tile_start, tile_cur, list, scan_total;
     DevBuf<double> red_partial, red_out;
@@ -267,6 +270,11 @@ int asm_set_option(asm_context* ctx, int option, int value)
         ctx->have_search = false;
         return ASM_OK;
     case ASM_OPT_DETERMINISTIC: ctx->deterministic = value ? 1 : 0; return ASM_OK;
+    case ASM_OPT_KNN_METHOD:
+        if (value != kKnnBoundCollect && value != kKnnInsertion)
+            return fail(ctx, ASM_ERR_ARG, "kNN method %d is not 0 or 1", value);
+        ctx->knn_method = value;
+        return ASM_OK;
     case ASM_OPT_TILE:
         if (value < 1 || value > 8) return fail(ctx, ASM_ERR_ARG, "tile %d outside 1..8", value);
         ctx->tile = value;
@@ -361,7 +369,17 @@ int asm_neighbor_search(asm_context* ctx, int k)
     HIPC(ctx->dens.ensure(n));
     HIPC(ctx->kkey.ensure(n));
     HIPC(ctx->kmax.ensure(static_cast<size_t>(2 * ctx->nleaf_pad)));
-    HIPC(ctx->kstats.ensure(1));
+    HIPC(ctx->kstats.ensure(kStatSlots));
+    const size_t nwaves = static_cast<size_t>((n + kWave - 1) / kWave);
+    HIPC(ctx->knn_flags.ensure(nwaves));
+    HIPC(ctx->knn_leaf_n.ensure(nwaves));
+    const bool bound_collect = ctx->knn_method == kKnnBoundCollect;
+    if (bound_collect) {
+        HIPC(ctx->knn_leaves.ensure(nwaves * kLeafCap));
+        HIPC(ctx->knn_tau.ensure(n));
+        if (!ctx->deterministic) HIPC(ctx->dens_pull.ensure(n));
+    }
+    const KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p};
     int32_t* nbr = nullptr;
     if (ctx->keep_neighbors) {
         HIPC(ctx->nbr.ensure(static_cast<size_t>(n) * kSlots));
@@ -370,11 +388,11 @@ int asm_neighbor_search(asm_context* ctx, int k)
     if (!ctx->deterministic) HIPC(ctx->dens64.ensure(n));
     {
         StageTimer t(ctx, T_KNN);
-        HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->kstats.p), sizeof(KnnStats) / 4, s));
+        HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->kstats.p), kStatSlots * sizeof(KnnStats) / 4, s));
         if (!ctx->deterministic) HIPC(launch_fill_zero_f64(ctx->dens64.p, n, s));
         HIPC(launch_knn(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->nleaf, ctx->leaf_size, ctx->rk.p,
                         ctx->kkey.p, ctx->inv_perm.p, ctx->deterministic ? nullptr : ctx->dens64.p, nbr, ctx->kstats.p,
-                        s));
+                        kscr, ctx->knn_method, s));
     }
     {
         StageTimer t(ctx, T_DENS);
@@ -383,10 +401,9 @@ int asm_neighbor_search(asm_context* ctx, int k)
             HIPC(launch_dens_gather(ctx->spos.p, ctx->kkey.p, ctx->rk.p, n, ctx->nodes.p, ctx->kmax.p,
                                     ctx->nleaf_pad, ctx->dens.p, s));
         } else {
-            HIPC(launch_dens_finalize(ctx->dens64.p, n, ctx->dens.p, s));
+            HIPC(launch_dens_finalize(ctx->dens64.p, bound_collect ? ctx->dens_pull.p : nullptr, n, ctx->dens.p, s));
         }
     }
-    HIPC(hipMemcpyAsync(&ctx->last_stats, ctx->kstats.p, sizeof(KnnStats), hipMemcpyDeviceToHost, s));
     ctx->have_search = true;
     return ASM_OK;
 }
@@ -702,11 +719,18 @@ int asm_get_search_stats(asm_context* ctx, uint64_t out[4])
 {
     if (!ctx || !out) return ASM_ERR_ARG;
     HIPC(hipSetDevice(ctx->device));
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (!ctx->have_search || !ctx->kstats.p) return ASM_OK;
+    std::vector<KnnStats> st(kStatSlots);
+    HIPC(hipMemcpyAsync(st.data(), ctx->kstats.p, kStatSlots * sizeof(KnnStats), hipMemcpyDeviceToHost,
+                        ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
-    out[0] = ctx->last_stats.node_tests;
-    out[1] = ctx->last_stats.leaf_scans;
-    out[2] = ctx->last_stats.dist_evals;
-    out[3] = ctx->last_stats.inserts;
+    for (const KnnStats& x : st) {
+        out[0] += x.node_tests;
+        out[1] += x.leaf_scans;
+        out[2] += x.dist_evals;
+        out[3] += x.inserts;
+    }
     return ASM_OK;
 }
 

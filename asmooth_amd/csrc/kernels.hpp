@@ -42,12 +42,34 @@ struct KnnStats {
     unsigned long long dist_evals;
     unsigned long long inserts;
 };
+// Search counters are spread over kStatSlots copies (wavefront w adds to
+// copy w % kStatSlots), so the per-wavefront atomics do not serialize on one
+// address; the host sums the copies.
+constexpr int kStatSlots = 1024;
 // kNN: rk = sqrtf(d2 of the 33rd key) and that key, per sorted particle.
-// With dens64 non-null the kernel also scatters the density terms (FP64 atomics).
+// With dens64 non-null the search also scatters the density terms (FP64
+// atomics).  method kKnnBoundCollect: bound walk + collect walk, with the
+// insertion kernel redoing flagged wavefronts (wave_flags: one uint32 per 64
+// particles); kKnnInsertion: the insertion kernel for every wavefront.
+constexpr int kKnnBoundCollect = 0;
+constexpr int kKnnInsertion = 1;
+constexpr int kLeafCap = 256;   // leaf ids recorded per wavefront by the bound walk
+// Device work buffers of the bound/collect method: per wavefront of 64
+// sorted particles, wleaf kLeafCap ids, wleaf_n and wave_flags one each;
+// per sorted particle, tau (33rd squared distance) and dens_pull (density
+// terms gathered by the particle itself; the rest arrive in dens64).
+struct KnnScratch {
+    uint32_t* wleaf;
+    uint32_t* wleaf_n;
+    uint32_t* wave_flags;
+    float* tau;
+    double* dens_pull;
+};
 hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
                       int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, const int32_t* inv_perm, double* dens64,
-                      int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s);
-hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s);
+                      int32_t* nbr_out_orig, KnnStats* stats, const KnnScratch& scratch, int method, hipStream_t s);
+// dens = dens64 (+ dens_pull when non-null), rounded to FP32.
+hipError_t launch_dens_finalize(const double* dens64, const double* dens_pull, int64_t n, float* dens, hipStream_t s);
 // Per-node maximum of the 33rd squared distance (2*nleaf_pad floats).
 hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t nleaf, int64_t nleaf_pad, float* kmax,
                        hipStream_t s);

@@ -7,25 +7,39 @@
 // One query per lane.  A wavefront takes 64 consecutive particles in Morton
 // order, so its queries are spatial neighbours, and walks the tree as a
 // packet: a node is opened when any lane's conservative box distance is
-// within that lane's current k-th distance; a leaf's particles are staged in
-// LDS and every lane tests every staged particle.  The traversal stack lives
-// in one VGPR (lane i holds entry i), so control flow stays wave-uniform and
-// node records come through the scalar cache.
+// within that lane's current bound; a leaf's particles are staged in LDS and
+// every lane tests every staged particle.  The traversal stack lives in one
+// VGPR (lane i holds entry i), so control flow stays wave-uniform and node
+// records come through the scalar cache.
 //
-// Each lane keeps its 33 smallest candidates as an ascending array of 64-bit
-// keys (FP32 squared distance bits, original index) in registers.  That is
-// the canonical (d2, index) order: ties are broken by index, and distances
-// use exactly the reference's FP32 arithmetic, so h and the neighbour sets
-// are reproducible bit for bit.  h = sqrtf(d2 of slot 32) (nbr_search.cpp:73).
+// The neighbour set is defined canonically: the 33 smallest keys (FP32
+// squared distance bits, original index), distances in exactly the
+// reference's FP32 arithmetic, ties broken by index; h = sqrtf(d2 of the
+// 33rd) (nbr_search.cpp:73).  It is found in two walks:
+//
+//  1. bound: each lane keeps only the 33 smallest squared distances, as an
+//     ascending float array in registers.  Inserting d into it is one
+//     v_med3_f32 per slot (slot i becomes med3(t[i-1], d, t[i])), with no
+//     compares or selects, and a lane whose d is not below t[32] is left
+//     unchanged by the same instructions.  At the end t[32] = tau, the 33rd
+//     smallest squared distance over all particles.
+//  2. collect: a second walk with tau fixed opens exactly the leaves whose
+//     box distance is within tau and appends every particle with d2 <= tau to
+//     a per-lane list in LDS: normally 33 entries, more only when distances
+//     tie at tau.  Entries below tau are the first slots in any order; ties at
+//     tau are ranked by index.
 //
 // The density scatter of nbr_search.cpp:78-84 (particle p adds its tent
-// kernel to its 32 nearest, itself included) runs at the end of the search
-// with native FP64 atomics (the FP32 result is the rounding of an FP64 sum,
-// so the summation order does not show).  A bitwise-reproducible mode
-// computes it instead as a gather: q receives W(p, q, h_p) from every p whose
-// 33rd key is above (d2(p,q), q); the kNN kernel stores each 33rd key and a
-// second walk, pruned by the largest 33rd distance below each node, sums the
-// terms in FP64 registers in a fixed order.
+// kernel to its 32 nearest, itself included) runs at the end of the collect
+// walk with native FP64 atomics (the FP32 result is the rounding of an FP64
+// sum, so the summation order does not show).  A wavefront whose tie list
+// overflows the LDS list is redone by the insertion kernel below, which keeps
+// the 33 (d2, index) keys sorted in registers (also selectable for every
+// wavefront).  A bitwise-reproducible mode computes the density instead as a
+// gather: q receives W(p, q, h_p) from every p whose 33rd key is above
+// (d2(p,q), q); the search stores each 33rd key and a second walk, pruned by
+// the largest 33rd distance below each node, sums the terms in FP64 registers
+// in a fixed order.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -33,6 +47,9 @@ namespace asm_hip {
 namespace {
 
 constexpr int kKnnThreads = 256;
+// Flags the collect walk leaves for the insertion kernel, per wavefront.
+constexpr uint32_t kRedoAll = 1u;    // nothing done: redo the search and the whole scatter
+constexpr uint32_t kRedoTies = 2u;   // terms below tau scattered: redo the search, scatter ties only
 constexpr int kWavesPerBlock = kKnnThreads / kWave;
 
 // Insert ck (known to be below key[K-1]) into the ascending array, dropping
@@ -60,6 +77,14 @@ __device__ __forceinline__ float tent_weight(float x0, float x1, float x2, float
     w = w * (1.0f - fabsf(x1 - y1) * hinv);
     w = w * (1.0f - fabsf(x2 - y2) * hinv);
     return w;
+}
+
+// The wavefront's index in its block, as a scalar: the compiler cannot tell
+// that threadIdx.x / 64 is uniform, and per-wavefront loop bounds derived
+// from it would otherwise live in VGPRs with divergent loop exits.
+__device__ __forceinline__ int wave_in_block()
+{
+    return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 }
 
 __device__ __forceinline__ void wave_lds_fence()
@@ -103,13 +128,16 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad,
     int64_t nleaf, int leaf_size, int init_leaves,
     float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted, const int32_t* __restrict__ inv_perm,
-    double* __restrict__ dens64, int32_t* __restrict__ nbr_out, KnnStats* __restrict__ stats)
+    double* __restrict__ dens64, int32_t* __restrict__ nbr_out, KnnStats* __restrict__ stats,
+    const uint32_t* __restrict__ only_flagged)
 {
     __shared__ float4 stage[kWavesPerBlock][kWave];
     const int lane = threadIdx.x % kWave;
-    const int wid = threadIdx.x / kWave;
+    const int wid = wave_in_block();
     const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid) * kWave;
     if (wave0 >= n) return;  // whole wavefront past the end
+    const uint32_t redo = only_flagged ? only_flagged[wave0 / kWave] : kRedoAll;
+    if (redo == 0u) return;  // done by the collect walk
     const int64_t q = wave0 + lane;
     const bool valid = q < n;
     const float4 me = spos[valid ? q : n - 1];
@@ -132,51 +160,54 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     int64_t l1 = last / leaf_size + init_leaves;
     if (l0 < 0) l0 = 0;
     if (l1 > nleaf - 1) l1 = nleaf - 1;
-    {
-        const int64_t p0 = l0 * leaf_size;
-        const int64_t p1 = ((l1 + 1) * leaf_size < n) ? (l1 + 1) * leaf_size : n;
-        scan_range<K>(spos, p0, p1 - p0, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
-        n_leaf += static_cast<unsigned long long>(l1 - l0 + 1);
-        n_dist += static_cast<unsigned long long>(p1 - p0);
-    }
-
-    int stack = 0;
+    // One call site of the scan (the seed range first, then each leaf the
+    // walk opens), so the compiler keeps a single copy of the key array.
+    int64_t r_start = l0 * leaf_size;
+    int64_t r_cnt = (((l1 + 1) * leaf_size < n) ? (l1 + 1) * leaf_size : n) - r_start;
+    n_leaf += static_cast<unsigned long long>(l1 - l0 + 1);
+    int stack = (lane == 0) ? 1 : 0;
     int sp = 1;
-    stack = (lane == 0) ? 1 : stack;
-    while (sp > 0) {
-        --sp;
-        const int node = __builtin_amdgcn_readlane(stack, sp);
-        const Box b = nodes[node];
-        ++n_node;
-        if (!wave_any(valid && box_d2(b, qx, qy, qz) <= kth)) continue;
-        if (node >= nleaf_pad) {
-            const int64_t leaf = node - nleaf_pad;
-            if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
-            ++n_leaf;
-            n_dist += static_cast<unsigned long long>(b.hi_w);
-            scan_range<K>(spos, b.lo_w, b.hi_w, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
-        } else {
-            const Box c0 = nodes[2 * node];
-            const Box c1 = nodes[2 * node + 1];
-            const float d0 = box_d2(c0, qx, qy, qz);
-            const float d1 = box_d2(c1, qx, qy, qz);
-            const bool need0 = wave_any(valid && d0 <= kth);
-            const bool need1 = wave_any(valid && d1 <= kth);
-            const int votes0 = __popcll(__ballot(valid && d0 <= d1));
-            const bool near0 = 2 * votes0 >= static_cast<int>(n_valid_lanes);
-            const int first = near0 ? 2 * node + 1 : 2 * node;   // pushed first = visited last
-            const int second = near0 ? 2 * node : 2 * node + 1;
-            const bool need_first = near0 ? need1 : need0;
-            const bool need_second = near0 ? need0 : need1;
-            if (need_first) {
-                stack = (lane == sp) ? first : stack;
-                ++sp;
-            }
-            if (need_second) {
-                stack = (lane == sp) ? second : stack;
-                ++sp;
+    for (;;) {
+        n_dist += static_cast<unsigned long long>(r_cnt);
+        scan_range<K>(spos, r_start, r_cnt, stage[wid], lane, qx, qy, qz, key, kth, n_ins);
+        bool found = false;
+        while (sp > 0 && !found) {
+            --sp;
+            const int node = __builtin_amdgcn_readlane(stack, sp);
+            const Box b = nodes[node];
+            ++n_node;
+            if (!wave_any(valid && box_d2(b, qx, qy, qz) <= kth)) continue;
+            if (node >= nleaf_pad) {
+                const int64_t leaf = node - nleaf_pad;
+                if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
+                ++n_leaf;
+                r_start = b.lo_w;
+                r_cnt = b.hi_w;
+                found = true;
+            } else {
+                const Box c0 = nodes[2 * node];
+                const Box c1 = nodes[2 * node + 1];
+                const float d0 = box_d2(c0, qx, qy, qz);
+                const float d1 = box_d2(c1, qx, qy, qz);
+                const bool need0 = wave_any(valid && d0 <= kth);
+                const bool need1 = wave_any(valid && d1 <= kth);
+                const int votes0 = __popcll(__ballot(valid && d0 <= d1));
+                const bool near0 = 2 * votes0 >= static_cast<int>(n_valid_lanes);
+                const int first = near0 ? 2 * node + 1 : 2 * node;   // pushed first = visited last
+                const int second = near0 ? 2 * node : 2 * node + 1;
+                const bool need_first = near0 ? need1 : need0;
+                const bool need_second = near0 ? need0 : need1;
+                if (need_first) {
+                    stack = (lane == sp) ? first : stack;
+                    ++sp;
+                }
+                if (need_second) {
+                    stack = (lane == sp) ? second : stack;
+                    ++sp;
+                }
             }
         }
+        if (!found) break;
     }
 
     if (valid) {
@@ -187,9 +218,13 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
         if (dens64) {
             // nbr_search.cpp:75-84: slots 0..K-2 (the particle itself and its
             // 31 nearest others) receive W(p, q, 1/h_p); FP64 atomic sums.
+            // After a collect walk that overflowed its lists, the terms
+            // below kd are already scattered; only ties at kd are left.
             const float hinv = 1.0f / h;
+            const bool ties_only = redo == kRedoTies;
 #pragma unroll
             for (int s = 0; s < K - 1; ++s) {
+                if (ties_only && key_d2(key[s]) < kd) continue;
                 const int32_t sq = inv_perm[key_idx(key[s])];
                 const float4 c = spos[sq];
                 unsafeAtomicAdd(&dens64[sq], static_cast<double>(tent_weight(qx, qy, qz, c.x, c.y, c.z, hinv)));
@@ -201,19 +236,449 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
             for (int s = 0; s < K; ++s) nbr_out[oq * K + s] = static_cast<int32_t>(key_idx(key[s]));
         }
     }
-    if (stats && lane == 0) {
-        atomicAdd(&stats->node_tests, n_node);
-        atomicAdd(&stats->leaf_scans, n_leaf);
-        atomicAdd(&stats->dist_evals, n_dist * n_valid_lanes);
-    }
     if (stats) {
         // per-lane insert counts summed over the wave
         unsigned long long v = n_ins;
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (lane == 0) atomicAdd(&stats->inserts, v);
+        KnnStats* const st = stats + ((wave0 / kWave) & (kStatSlots - 1));
+        if (lane == 0) {
+            atomicAdd(&st->node_tests, n_node);
+            atomicAdd(&st->leaf_scans, n_leaf);
+            atomicAdd(&st->dist_evals, n_dist * n_valid_lanes);
+            atomicAdd(&st->inserts, v);
+        }
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Bound walk and collect walk.
+
+constexpr int kBoundThreads = 256;
+constexpr int kBoundWaves = kBoundThreads / kWave;
+constexpr int kCollectThreads = 128;
+constexpr int kCollectWaves = kCollectThreads / kWave;
+constexpr int kCollectSlots = 40;   // per-lane list of all entries: 33 plus up to 7 ties at tau
+constexpr int kTieSlots = 8;        // per-lane list of the entries at tau
+
+// Insert d into the ascending array t, dropping the largest entry: slot i
+// becomes the median of (t[i-1], d, t[i]), slot 0 the median of (0, d, t[0])
+// (all values are >= 0).  A d that is not below t[K-1] leaves every slot
+// unchanged.
+template <int K>
+__device__ __forceinline__ void med3_insert(float (&t)[K], float d)
+{
+#pragma unroll
+    for (int i = K - 1; i > 0; --i) t[i] = __builtin_amdgcn_fmed3f(t[i - 1], d, t[i]);
+    t[0] = __builtin_amdgcn_fmed3f(0.0f, d, t[0]);
+}
+
+// Stage up to 64 particles of [start, start + cnt) in LDS; slots past the
+// end hold a point at infinity (squared distance +inf: never accepted), so
+// the scan loops can take four candidates per step.
+__device__ __forceinline__ int stage_batch(const float4* __restrict__ spos, int64_t first, int64_t left,
+                                           float4* stage, int lane)
+{
+    const int m = static_cast<int>(left < kWave ? left : kWave);
+    stage[lane] = (lane < m) ? spos[first + lane] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    wave_lds_fence();
+    return (m + 3) & ~3;
+}
+
+template <int K>
+__device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int64_t start, int64_t cnt,
+                                           float4* stage, int lane, float qx, float qy, float qz, float (&t)[K],
+                                           uint32_t& n_acc)
+{
+    for (int64_t base = 0; base < cnt; base += kWave) {
+        const int m4 = stage_batch(spos, start + base, cnt - base, stage, lane);
+        for (int j = 0; j < m4; j += 4) {
+            float d[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float4 c = stage[j + u];
+                d[u] = norm2(qx, qy, qz, c.x, c.y, c.z);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool acc = d[u] < t[K - 1];
+                if (__ballot(acc)) {
+                    med3_insert<K>(t, d[u]);
+                    n_acc += acc ? 1u : 0u;
+                }
+            }
+        }
+        wave_lds_fence();
+    }
+}
+
+// Walk 1: tau[q] = 33rd smallest squared distance from q (itself included).
+// Seeded from the wavefront's own leaves and init_leaves leaves on either
+// side in Morton order, which are then skipped by the walk.  A node is opened
+// when some lane's box distance is within its current t[K-1] (>= tau), so
+// the leaves opened here include every leaf that walk 2 needs; their ids are
+// recorded per wavefront (up to kLeafCap; beyond that the count alone is
+// kept and walk 2 hands the wavefront to the insertion kernel).
+template <int K>
+__global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
+    const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
+    int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
+    uint32_t* __restrict__ wleaf_n, KnnStats* __restrict__ stats)
+{
+    __shared__ float4 stage[kBoundWaves][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wid = wave_in_block();
+    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kBoundWaves + wid) * kWave;
+    if (wave0 >= n) return;
+    const int64_t wave = wave0 / kWave;
+    const int64_t q = wave0 + lane;
+    const bool valid = q < n;
+    const float4 me = spos[valid ? q : n - 1];
+    const float qx = me.x, qy = me.y, qz = me.z;
+
+    float t[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) t[i] = INFINITY;
+
+    unsigned long long n_node = 0, n_leaf = 0, n_dist = 0;
+    uint32_t n_acc = 0;
+    const unsigned long long n_valid_lanes = __popcll(__ballot(valid));
+
+    const int64_t last = (n < wave0 + kWave ? n : wave0 + kWave) - 1;
+    int64_t l0 = wave0 / leaf_size - init_leaves;
+    int64_t l1 = last / leaf_size + init_leaves;
+    if (l0 < 0) l0 = 0;
+    if (l1 > nleaf - 1) l1 = nleaf - 1;
+
+    // Recorded leaf ids: 64 at a time in one VGPR (lane i holds entry i),
+    // flushed with one coalesced store.
+    uint32_t rec = 0;
+    int n_rec = 0;
+    uint32_t* const my_leaves = wleaf + wave * kLeafCap;
+
+    // One call site of the scan (the seed range first, then each leaf the
+    // walk opens), so the compiler keeps a single copy of t.
+    int64_t r_start = l0 * leaf_size;
+    int64_t r_cnt = (((l1 + 1) * leaf_size < n) ? (l1 + 1) * leaf_size : n) - r_start;
+    n_leaf += static_cast<unsigned long long>(l1 - l0 + 1);
+    int stack = (lane == 0) ? 1 : 0;
+    int sp = 1;
+    for (;;) {
+        n_dist += static_cast<unsigned long long>(r_cnt);
+        bound_scan<K>(spos, r_start, r_cnt, stage[wid], lane, qx, qy, qz, t, n_acc);
+        bool found = false;
+        while (sp > 0 && !found) {
+            --sp;
+            const int node = __builtin_amdgcn_readlane(stack, sp);
+            const Box b = nodes[node];
+            ++n_node;
+            const float kth = t[K - 1];
+            if (!wave_any(valid && box_d2(b, qx, qy, qz) <= kth)) continue;
+            if (node >= nleaf_pad) {
+                const int64_t leaf = node - nleaf_pad;
+                if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
+                ++n_leaf;
+                r_start = b.lo_w;
+                r_cnt = b.hi_w;
+                found = true;
+                if (n_rec < kLeafCap) {
+                    rec = (lane == (n_rec & (kWave - 1))) ? static_cast<uint32_t>(leaf) : rec;
+                    if ((n_rec & (kWave - 1)) == kWave - 1) my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
+                }
+                ++n_rec;
+            } else {
+                const Box c0 = nodes[2 * node];
+                const Box c1 = nodes[2 * node + 1];
+                const float d0 = box_d2(c0, qx, qy, qz);
+                const float d1 = box_d2(c1, qx, qy, qz);
+                const bool need0 = wave_any(valid && d0 <= kth);
+                const bool need1 = wave_any(valid && d1 <= kth);
+                const int votes0 = __popcll(__ballot(valid && d0 <= d1));
+                const bool near0 = 2 * votes0 >= static_cast<int>(n_valid_lanes);
+                const int first = near0 ? 2 * node + 1 : 2 * node;   // pushed first = visited last
+                const int second = near0 ? 2 * node : 2 * node + 1;
+                const bool need_first = near0 ? need1 : need0;
+                const bool need_second = near0 ? need0 : need1;
+                if (need_first) {
+                    stack = (lane == sp) ? first : stack;
+                    ++sp;
+                }
+                if (need_second) {
+                    stack = (lane == sp) ? second : stack;
+                    ++sp;
+                }
+            }
+        }
+        if (!found) break;
+    }
+    if (n_rec <= kLeafCap && (n_rec & (kWave - 1)) != 0 && lane < (n_rec & (kWave - 1)))
+        my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
+    if (lane == 0) wleaf_n[wave] = static_cast<uint32_t>(n_rec);
+    if (valid) tau_out[q] = t[K - 1];
+
+    if (stats) {
+        unsigned long long v = n_acc;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        KnnStats* const st = stats + (wave & (kStatSlots - 1));
+        if (lane == 0) {
+            atomicAdd(&st->node_tests, n_node);
+            atomicAdd(&st->leaf_scans, n_leaf);
+            atomicAdd(&st->dist_evals, n_dist * n_valid_lanes);
+            atomicAdd(&st->inserts, v);
+        }
+    }
+}
+
+// LDS of one wavefront in the collect walk.
+template <int M>
+struct CollectLds {
+    uint32_t list[kWave][M + 1];  // per-lane lists; row stride M + 1 keeps rows in distinct banks
+    double pacc[kWave];           // density terms pulled by each lane
+    float4 qtab[kWave];           // participating lanes of a leaf: (x, y, z, tau) ...
+    float htab[kWave];            // ... their 1/h ...
+    uint32_t otab[kWave];         // ... and their lane ids
+    uint32_t lcnt[kWave];         // entries within tau
+    uint32_t ecnt[kWave];         // entries at exactly tau
+    uint32_t tidx[kWave];         // original index of one entry at tau
+};
+
+// One leaf of the collect walk.  Only the lanes in `mask` (box distance
+// within their tau) take part, and the (lane, particle) pairs are packed
+// onto the wavefront: with the leaf padded to P = 64 / G particles, lane l
+// tests particle l % P for the participating lanes of rank l / P, l / P + G,
+// ...  An accepted pair (d2 <= tau_o for owner o and particle c) is counted
+// and listed for o, and its density terms are settled:
+//   * c's term on o (o is in c's first 32 slots when d2 < tau_c) is pulled
+//     by o into its LDS accumulator;
+//   * o's term on c (c in o's first 32 slots when d2 < tau_o) is pulled by
+//     c in its own walk when c sees o (d2 <= tau_c), so it is scattered with
+//     a global atomic only when d2 > tau_c;
+//   * terms at exactly tau_o depend on index order and are left to the end.
+template <int M, int P, bool kAll>
+__device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int count, uint64_t mask, float4 c,
+                                             float tc, int lane, float qx, float qy, float qz, float tau, float hinv,
+                                             double* __restrict__ dens64, unsigned long long& n_pairs)
+{
+    constexpr int G = kWave / P;
+    const int j = lane % P;
+    const int grp = lane / P;
+    const int a = __popcll(mask);
+    n_pairs += static_cast<unsigned long long>(a) * static_cast<unsigned long long>(count);
+    if ((mask >> lane) & 1u) {
+        const int k = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+        L.qtab[k] = make_float4(qx, qy, qz, tau);
+        L.htab[k] = hinv;
+        L.otab[k] = static_cast<uint32_t>(lane);
+    }
+    wave_lds_fence();
+    const float hc = 1.0f / sqrtf(tc);   // c's 1/h, as c's own lane computes it
+    for (int k = grp; k < a; k += G) {
+        const float4 o = L.qtab[k];
+        const float d2 = norm2(o.x, o.y, o.z, c.x, c.y, c.z);
+        if (d2 <= o.w) {
+            const uint32_t own = L.otab[k];
+            const uint32_t slot = atomicAdd(&L.lcnt[own], 1u);
+            if (kAll && slot < static_cast<uint32_t>(M)) L.list[own][slot] = static_cast<uint32_t>(start + j);
+            if (d2 < o.w) {
+                if (dens64 && d2 > tc)
+                    unsafeAtomicAdd(&dens64[start + j],
+                                    static_cast<double>(tent_weight(o.x, o.y, o.z, c.x, c.y, c.z, L.htab[k])));
+            } else {
+                const uint32_t e = atomicAdd(&L.ecnt[own], 1u);
+                L.tidx[own] = __float_as_uint(c.w);
+                if (!kAll && e < static_cast<uint32_t>(M)) L.list[own][e] = static_cast<uint32_t>(start + j);
+            }
+            if (dens64 && d2 < tc)
+                atomicAdd(&L.pacc[own], static_cast<double>(tent_weight(c.x, c.y, c.z, o.x, o.y, o.z, hc)));
+        }
+    }
+    wave_lds_fence();
+}
+
+// Walk 2: with tau fixed (from walk 1), visit every leaf whose box distance
+// is within some lane's tau, count and list each lane's entries with
+// d2 <= tau, settle the density terms (collect_leaf), then derive h, the
+// 33rd key, the terms at tau and (optionally) the ordered neighbour list.
+// Lists hold every entry (kAll, when the ordered lists are wanted) or only
+// the entries at tau.
+//
+// The leaves are the seed range and those recorded by walk 1, taken 64 at a
+// time: lane i loads the box of item i, each box is broadcast to test every
+// lane, and the particles of the next participating leaf are loaded while
+// the current one is processed.  A wavefront whose walk 1 opened more leaves
+// than were recorded walks the tree again instead.  A lane with more
+// entries than its list holds sends the wavefront to the insertion kernel,
+// which redoes the search and scatters the terms at tau only (everything
+// else is settled here).
+template <int K, int M, int P, bool kAll>
+__global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
+    const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
+    int leaf_size, int init_leaves, const uint32_t* __restrict__ wleaf, const uint32_t* __restrict__ wleaf_n,
+    const float* __restrict__ tau_sorted, float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted,
+    double* __restrict__ dens64, double* __restrict__ dens_pull, int32_t* __restrict__ nbr_out,
+    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats)
+{
+    __shared__ CollectLds<M> lds[kCollectWaves];
+    const int lane = threadIdx.x % kWave;
+    const int wid = wave_in_block();
+    CollectLds<M>& L = lds[wid];
+    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kCollectWaves + wid) * kWave;
+    if (wave0 >= n) return;
+    const int64_t wave = wave0 / kWave;
+    const uint32_t n_rec = wleaf_n[wave];
+    const int64_t q = wave0 + lane;
+    const bool valid = q < n;
+    const float4 me = spos[valid ? q : n - 1];
+    const float qx = me.x, qy = me.y, qz = me.z;
+    // Squared distances and box distances are >= 0, so a past-the-end lane
+    // with tau = -1 opens and accepts nothing.
+    const float tau = valid ? tau_sorted[q] : -1.0f;
+    const float h = sqrtf(tau);
+    const float hinv = 1.0f / h;
+    L.lcnt[lane] = 0u;
+    L.ecnt[lane] = 0u;
+    L.pacc[lane] = 0.0;
+    wave_lds_fence();
+
+    const int j = lane % P;
+    unsigned long long n_leaf = 0, n_pairs = 0;
+    if (n_rec <= static_cast<uint32_t>(kLeafCap)) {
+        const int64_t last = (n < wave0 + kWave ? n : wave0 + kWave) - 1;
+        int64_t l0 = wave0 / leaf_size - init_leaves;
+        int64_t l1 = last / leaf_size + init_leaves;
+        if (l0 < 0) l0 = 0;
+        if (l1 > nleaf - 1) l1 = nleaf - 1;
+        const int64_t n_seed = l1 - l0 + 1;
+        const uint32_t* const my_leaves = wleaf + wave * kLeafCap;
+        const int64_t n_items = n_seed + static_cast<int64_t>(n_rec);
+        for (int64_t base = 0; base < n_items; base += kWave) {
+            const int nb = static_cast<int>(n_items - base < kWave ? n_items - base : kWave);
+            Box mb;
+            {
+                const int64_t my_it = base + (lane < nb ? lane : 0);
+                const int64_t leaf = my_it < n_seed ? l0 + my_it : static_cast<int64_t>(my_leaves[my_it - n_seed]);
+                mb = nodes[nleaf_pad + leaf];
+            }
+            uint32_t m_lo = 0, m_hi = 0;   // lane i: participation mask of item base + i
+            for (int i = 0; i < nb; ++i) {
+                Box bb;
+                bb.lo[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mb.lo[0]), i));
+                bb.lo[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mb.lo[1]), i));
+                bb.lo[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mb.lo[2]), i));
+                bb.hi[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mb.hi[0]), i));
+                bb.hi[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mb.hi[1]), i));
+                bb.hi[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mb.hi[2]), i));
+                const uint64_t mk = __ballot(box_d2(bb, qx, qy, qz) <= tau);
+                m_lo = (lane == i) ? static_cast<uint32_t>(mk) : m_lo;
+                m_hi = (lane == i) ? static_cast<uint32_t>(mk >> 32) : m_hi;
+            }
+            uint64_t part = __ballot((m_lo | m_hi) != 0u);
+            if (part == 0) continue;
+            int nxt = __builtin_ctzll(part);
+            int nstart = __builtin_amdgcn_readlane(mb.lo_w, nxt);
+            int ncount = __builtin_amdgcn_readlane(mb.hi_w, nxt);
+            float4 cn = (j < ncount) ? spos[nstart + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+            float tn = (j < ncount) ? tau_sorted[nstart + j] : 0.0f;
+            while (part != 0) {
+                const int cur = nxt;
+                const int start = nstart;
+                const int count = ncount;
+                const float4 c = cn;
+                const float tc = tn;
+                part &= part - 1;
+                if (part != 0) {
+                    nxt = __builtin_ctzll(part);
+                    nstart = __builtin_amdgcn_readlane(mb.lo_w, nxt);
+                    ncount = __builtin_amdgcn_readlane(mb.hi_w, nxt);
+                    cn = (j < ncount) ? spos[nstart + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+                    tn = (j < ncount) ? tau_sorted[nstart + j] : 0.0f;
+                }
+                const uint64_t mask = (static_cast<uint64_t>(__builtin_amdgcn_readlane(m_hi, cur)) << 32) |
+                                      static_cast<uint32_t>(__builtin_amdgcn_readlane(m_lo, cur));
+                ++n_leaf;
+                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64,
+                                         n_pairs);
+            }
+        }
+    } else {
+        // More leaves than walk 1 recorded: walk the tree with tau.
+        int stack = (lane == 0) ? 1 : 0;
+        int sp = 1;
+        while (sp > 0) {
+            --sp;
+            const int node = __builtin_amdgcn_readlane(stack, sp);
+            const Box b = nodes[node];
+            const uint64_t mask = __ballot(box_d2(b, qx, qy, qz) <= tau);
+            if (mask == 0) continue;
+            if (node >= nleaf_pad) {
+                ++n_leaf;
+                const int start = b.lo_w;
+                const int count = b.hi_w;
+                const float4 c = (j < count) ? spos[start + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+                const float tc = (j < count) ? tau_sorted[start + j] : 0.0f;
+                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64, n_pairs);
+            } else {
+                stack = (lane == sp) ? 2 * node + 1 : stack;
+                ++sp;
+                stack = (lane == sp) ? 2 * node : stack;
+                ++sp;
+            }
+        }
+    }
+    wave_lds_fence();
+    const int cnt = static_cast<int>(L.lcnt[lane]);
+    const int c_eq = static_cast<int>(L.ecnt[lane]);
+    if (dens64 && valid) dens_pull[q] = L.pacc[lane];
+
+    if (stats && lane == 0) {
+        KnnStats* const st = stats + (wave & (kStatSlots - 1));
+        atomicAdd(&st->leaf_scans, n_leaf);
+        atomicAdd(&st->dist_evals, n_pairs);
+    }
+    const int n_list = kAll ? cnt : c_eq;
+    const bool over = wave_any(n_list > M);
+    if (lane == 0) wave_flags[wave] = over ? kRedoTies : 0u;
+    if (over || !valid) return;
+
+    rk_sorted[q] = h;
+    // Entries below tau are slots 0 .. c_lt-1 (c_lt <= 32); the c_eq >= 1
+    // entries at tau follow in index order.
+    const int c_lt = cnt - c_eq;
+    uint64_t kk = make_key(tau, L.tidx[lane]);
+    if (c_eq > 1) {
+        // The first 32 - c_lt ties are scattered, the one at slot 32 is the
+        // 33rd key.
+        for (int i = 0; i < n_list; ++i) {
+            const uint32_t si = L.list[lane][i];
+            const float4 c = spos[si];
+            if (norm2(qx, qy, qz, c.x, c.y, c.z) < tau) continue;
+            const uint32_t idx = __float_as_uint(c.w);
+            int r = c_lt;
+            for (int k = 0; k < n_list; ++k) {
+                const float4 o = spos[L.list[lane][k]];
+                if (norm2(qx, qy, qz, o.x, o.y, o.z) == tau && __float_as_uint(o.w) < idx) ++r;
+            }
+            if (r < K - 1 && dens64)
+                unsafeAtomicAdd(&dens64[si], static_cast<double>(tent_weight(qx, qy, qz, c.x, c.y, c.z, hinv)));
+            if (r == K - 1) kk = make_key(tau, idx);
+        }
+    }
+    kkey_sorted[q] = kk;
+    if (kAll && nbr_out) {
+        const int64_t oq = static_cast<int64_t>(__float_as_uint(me.w));
+        for (int i = 0; i < cnt; ++i) {
+            const float4 c = spos[L.list[lane][i]];
+            const uint64_t ki = make_key(norm2(qx, qy, qz, c.x, c.y, c.z), __float_as_uint(c.w));
+            int r = 0;
+            for (int k = 0; k < cnt; ++k) {
+                const float4 o = spos[L.list[lane][k]];
+                if (make_key(norm2(qx, qy, qz, o.x, o.y, o.z), __float_as_uint(o.w)) < ki) ++r;
+            }
+            if (r < K) nbr_out[oq * K + r] = static_cast<int32_t>(__float_as_uint(c.w));
+        }
+    }
+}
 
 // Largest 33rd squared distance over the particles below each node.
 __global__ void kmax_leaves_kernel(const uint64_t* __restrict__ kkey, int64_t n, int leaf_size, int64_t nleaf,
@@ -249,7 +714,7 @@ __global__ __launch_bounds__(kKnnThreads) void dens_gather_kernel(
     __shared__ float4 stage_p[kWavesPerBlock][kWave];
     __shared__ float4 stage_k[kWavesPerBlock][kWave];   // (kkey lo, kkey hi, hinv, -)
     const int lane = threadIdx.x % kWave;
-    const int wid = threadIdx.x / kWave;
+    const int wid = wave_in_block();
     const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid) * kWave;
     if (wave0 >= n) return;
     const int64_t q = wave0 + lane;
@@ -298,32 +763,65 @@ __global__ __launch_bounds__(kKnnThreads) void dens_gather_kernel(
     if (valid) dens[q] = static_cast<float>(acc);
 }
 
-__global__ void dens_finalize_kernel(const double* __restrict__ d64, int64_t n, float* __restrict__ d)
+__global__ void dens_finalize_kernel(const double* __restrict__ d64, const double* __restrict__ pull, int64_t n,
+                                     float* __restrict__ d)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = static_cast<float>(d64[i]);
+    if (i < n) d[i] = static_cast<float>(pull ? d64[i] + pull[i] : d64[i]);
 }
 
 }  // namespace
 
-hipError_t launch_dens_finalize(const double* dens64, int64_t n, float* dens, hipStream_t s)
+hipError_t launch_dens_finalize(const double* dens64, const double* dens_pull, int64_t n, float* dens, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    dens_finalize_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(dens64, n, dens);
+    dens_finalize_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(dens64, dens_pull, n, dens);
     return hipGetLastError();
 }
 
 hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
                       int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, const int32_t* inv_perm, double* dens64,
-                      int32_t* nbr_out_orig, KnnStats* stats, hipStream_t s)
+                      int32_t* nbr_out_orig, KnnStats* stats, const KnnScratch& scratch, int method, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int64_t waves = (n + kWave - 1) / kWave;
-    const unsigned blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     const int init_leaves = leaf_size >= kWave ? 1 : kWave / leaf_size;
-    knn_dens_kernel<kSlots><<<blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves,
-                                                          rk_sorted, kkey_sorted, inv_perm, dens64, nbr_out_orig,
-                                                          stats);
+    const unsigned ins_blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (method == kKnnInsertion) {
+        knn_dens_kernel<kSlots><<<ins_blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size,
+                                                                  init_leaves, rk_sorted, kkey_sorted, inv_perm,
+                                                                  dens64, nbr_out_orig, stats, nullptr);
+        return hipGetLastError();
+    }
+    if (leaf_size < 1 || leaf_size > kWave) return hipErrorInvalidValue;
+    const unsigned bound_blocks = static_cast<unsigned>((waves + kBoundWaves - 1) / kBoundWaves);
+    knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size,
+                                                                   init_leaves, scratch.tau, scratch.wleaf,
+                                                                   scratch.wleaf_n, stats);
+    ASM_HIP_CHECK(hipGetLastError());
+    const unsigned col_blocks = static_cast<unsigned>((waves + kCollectWaves - 1) / kCollectWaves);
+#define ASM_COLLECT(M, PAD, ALL)                                                                                \
+    knn_collect_kernel<kSlots, M, PAD, ALL><<<col_blocks, kCollectThreads, 0, s>>>(                             \
+        spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.wleaf, scratch.wleaf_n, scratch.tau,  \
+        rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats)
+#define ASM_COLLECT_PAD(M, ALL)                  \
+    if (leaf_size <= 8) ASM_COLLECT(M, 8, ALL);  \
+    else if (leaf_size <= 16) ASM_COLLECT(M, 16, ALL); \
+    else if (leaf_size <= 32) ASM_COLLECT(M, 32, ALL); \
+    else ASM_COLLECT(M, 64, ALL);
+    if (nbr_out_orig) {
+        ASM_COLLECT_PAD(kCollectSlots, true)
+    } else {
+        ASM_COLLECT_PAD(kTieSlots, false)
+    }
+#undef ASM_COLLECT_PAD
+#undef ASM_COLLECT
+    ASM_HIP_CHECK(hipGetLastError());
+    // Wavefronts flagged by the collect walk are redone by the insertion
+    // kernel; every other wavefront returns at once.
+    knn_dens_kernel<kSlots><<<ins_blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size,
+                                                              init_leaves, rk_sorted, kkey_sorted, inv_perm, dens64,
+                                                              nbr_out_orig, stats, scratch.wave_flags);
     return hipGetLastError();
 }
 

@@ -104,6 +104,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-side", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--knn-method", type=int, default=0,
+                    help="0: bound walk + collect walk (default); 1: sorted-key insertion lists")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU). "
@@ -123,6 +125,7 @@ def main(argv=None) -> int:
     ctx = A.Context(0)
     ctx.set_particles(parts, nl, L)
     ctx.set_option(A.OPT_PROFILE, 1)
+    ctx.set_option(A.OPT_KNN_METHOD, args.knn_method)
     for i in range(args.warmup):
         run_step(ctx, L, bf, ncs)
         ctx.synchronize()
@@ -167,7 +170,7 @@ def main(argv=None) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"zeldovich {n_side}^3 L={L:g} bf={bf} k=32 nc={','.join(map(str, ncs))}",
-                   "n_local": nl, "n_buf": nbuf, "ncs": ncs, "leaf_size": 32, "tile": 8,
+                   "n_local": nl, "n_buf": nbuf, "ncs": ncs, "leaf_size": 32, "tile": 8, "knn_method": args.knn_method,
                    "path_roofline_frac": round(b_alg / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 6),
                    "mass_check_ok": bool(mass_ok),
                    "stage_ms": {k: round(v[0] / args.steps, 4) for k, v in timers.items() if v[1]},

@@ -66,8 +66,10 @@ enum {
     ASM_OPT_PROFILE = 2,        /* 1: time every kernel with HIP events */
     ASM_OPT_LEAF_SIZE = 3,      /* particles per tree leaf, 8..64 (default 32) */
     ASM_OPT_TILE = 4,           /* deposition tile side in cells, 1..8 (default 8) */
-    ASM_OPT_DETERMINISTIC = 5   /* 1: density by an ordered gather, bitwise reproducible
+    ASM_OPT_DETERMINISTIC = 5,  /* 1: density by an ordered gather, bitwise reproducible
                                       run to run (default 0: FP64 atomic scatter) */
+    ASM_OPT_KNN_METHOD = 6      /* 0: bound walk + collect walk (default); 1: sorted-key
+                                      insertion lists for every wavefront.  Same results. */
 };
 
 const char* asm_version(void);

@@ -48,6 +48,22 @@ def test_path_matches_oracle(ctx, kind, n_side, seed):
         assert abs(tn - on) <= 1e-9 * abs(on) and th == oh
 
 
+@pytest.mark.parametrize("kind,n_side,seed", [("zeldovich", 16, 1), ("lattice", 10, 2), ("blobs", 20, 8)])
+def test_path_without_kept_lists_matches_oracle(ctx, kind, n_side, seed):
+    # the configuration the benchmark runs: no neighbour lists kept
+    xv, L, bf = small_set(kind, n_side, seed)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    ncs = [8, n_side]
+    o = oracle_path(buf, L, ncs)
+    g = gpu_path(ctx, parts, nl, L, bf, ncs, keep_neighbors=False)
+    assert np.array_equal(g["rk"].view(np.int32), o["rk"].view(np.int32))
+    assert rel_err(g["dens"], o["dens64"]) <= DENS_RTOL
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    for nc in ncs:
+        _check_mesh(g["meshes"][nc], o["meshes"][nc], nc, vmax)
+
+
 @pytest.mark.parametrize("leaf_size", [8, 16, 64])
 def test_leaf_size_does_not_change_results(ctx, leaf_size):
     xv, L, bf = small_set("blobs", 16, 7)
@@ -89,6 +105,82 @@ def test_minimum_size_and_duplicates(ctx):
     assert np.array_equal(rk, ork)
     assert np.array_equal(nbr, onbr)
     assert rel_err(dens, O.density(pos, onbr, ork)) <= DENS_RTOL
+
+
+def _knn_run(ctx, pos, method, deterministic=0, keep=1):
+    parts = A.particles_from_xv(np.concatenate([pos, np.zeros_like(pos)], 1))
+    ctx.set_option(A.OPT_KEEP_NEIGHBORS, keep)
+    ctx.set_option(A.OPT_KNN_METHOD, method)
+    ctx.set_option(A.OPT_DETERMINISTIC, deterministic)
+    try:
+        lo, hi = float(pos.min()) - 0.5, float(pos.max()) + 0.5
+        ctx.set_particles(parts, len(pos), hi - lo)
+        ctx.construct(lo, hi)
+        ctx.neighbor_search(32)
+        rk, dens, _ = ctx.get_rk_dens()
+        return rk, dens, (ctx.get_neighbors() if keep else None)
+    finally:
+        ctx.set_option(A.OPT_KNN_METHOD, 0)
+        ctx.set_option(A.OPT_DETERMINISTIC, 0)
+
+
+def _tie_heavy_sets():
+    # * quantized30: 2304 distinct points of a 30^3 integer grid.  Squared
+    #   distances are integers, so most particles have several neighbours
+    #   tied at the 33rd distance and only some of them fit in the 33 slots
+    #   (ranked by index); about a fifth of the wavefronts have a lane with
+    #   more than 40 particles within that distance and are redone by the
+    #   insertion kernel.
+    # * lattice7: an exact 7^3 lattice, up to 20 particles tied at the 33rd
+    #   distance near corners and edges.
+    # * coincident50: 50 coincident particles (50 ties at d2 = 0).
+    rng = np.random.default_rng(3)
+    quant = np.unique(rng.integers(0, 30, (2400, 3)), axis=0).astype(np.float32)
+    rng.shuffle(quant)
+    g = np.arange(7, dtype=np.float32)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).copy()
+    dup = rng.uniform(0, 6, (300, 3)).astype(np.float32)
+    dup[100:150] = dup[100]
+    return {"quantized30": quant, "lattice7": lat, "coincident50": dup}
+
+
+@pytest.mark.parametrize("name", ["quantized30", "lattice7", "coincident50"])
+@pytest.mark.parametrize("method", [0, 1])
+def test_knn_ties_match_oracle(ctx, name, method):
+    pos = _tie_heavy_sets()[name]
+    onbr, _, ork = O.knn(pos)
+    odens = O.density(pos, onbr, ork)
+    rk, dens, nbr = _knn_run(ctx, pos, method)
+    assert np.array_equal(rk.view(np.int32), ork.view(np.int32))
+    assert np.array_equal(nbr, onbr)
+    # 33 or more coincident particles have h = 0, and their tent terms are
+    # 0 * inf = NaN, as in the reference (nbr_search.cpp:27-37): NaN at the
+    # same particles, the rest to the usual tolerance.
+    nan = np.isnan(odens)
+    assert np.array_equal(np.isnan(dens), nan)
+    assert rel_err(dens[~nan], odens[~nan]) <= DENS_RTOL
+    # without kept lists (the bench configuration): same h and density
+    rk2, dens2, _ = _knn_run(ctx, pos, method, keep=0)
+    assert np.array_equal(rk2.view(np.int32), ork.view(np.int32))
+    assert np.array_equal(np.isnan(dens2), nan)
+    assert rel_err(dens2[~nan], odens[~nan]) <= DENS_RTOL
+    # the 33rd keys drive the ordered gather: same density to FP32 rounding
+    for keep in (1, 0):
+        _, dens_g, _ = _knn_run(ctx, pos, method, deterministic=1, keep=keep)
+        assert np.array_equal(np.isnan(dens_g), nan)
+        assert rel_err(dens_g[~nan], odens[~nan]) <= DENS_RTOL
+
+
+@pytest.mark.parametrize("kind,n_side,seed", [("zeldovich", 20, 4), ("blobs", 20, 6)])
+def test_knn_methods_agree(ctx, kind, n_side, seed):
+    xv, L, bf = small_set(kind, n_side, seed)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    pos = np.ascontiguousarray(parts["x"])
+    r0 = _knn_run(ctx, pos, 0)
+    r1 = _knn_run(ctx, pos, 1)
+    assert np.array_equal(r0[0].view(np.int32), r1[0].view(np.int32))
+    assert np.array_equal(r0[2], r1[2])
+    assert rel_err(r0[1], r1[1]) <= DENS_RTOL
 
 
 def test_too_few_particles_is_an_error(ctx):
