@@ -22,6 +22,13 @@ constexpr int kTileMax = 8;                 // cells per tile side (LDS sized fo
 constexpr int kTileCells = kTileMax * kTileMax * kTileMax;
 constexpr int kDepThreads = 256;
 
+__device__ __forceinline__ void wave_lds_fence_dep()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // coarse_mesh.cpp:10-15.
 __device__ __forceinline__ float tent_f(float x)
 {
@@ -221,6 +228,41 @@ __global__ void deposit_fill_kernel(DepositArgs a, const uint32_t* __restrict__ 
     tile_pass(a, tile_cur, tile_start, list);
 }
 
+// Write one tile's FP64 accumulators to the mesh (FP32), every cell once.
+__device__ __forceinline__ void flush_tile(const DepositArgs& a, const double (*acc)[kTileCells], int cx0, int cy0,
+                                           int cz0, float* __restrict__ mesh, int accumulate)
+{
+    const int tid = threadIdx.x;
+    const int T = a.tile;
+    const int ncell = T * T * T;
+    const int64_t nc = a.nc;
+    const int64_t nmesh = nc * nc * nc;
+    for (int c = tid; c < ncell; c += kDepThreads) {
+        const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
+        const int gx = cx0 + lx, gy = cy0 + ly, gz = cz0 + lz;
+        if (gx >= a.nc || gy >= a.nc || gz >= a.nc) continue;
+        const int64_t idx = (static_cast<int64_t>(gz) * nc + gy) * nc + gx;
+        float vals[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) vals[f] = static_cast<float>(acc[f][c]);
+        if (accumulate) {
+            mesh[idx] += vals[0];
+            mesh[nmesh + idx] += vals[1];
+            mesh[2 * nmesh + idx] += vals[2];
+            mesh[3 * nmesh + 3 * idx + 0] += vals[3];
+            mesh[3 * nmesh + 3 * idx + 1] += vals[4];
+            mesh[3 * nmesh + 3 * idx + 2] += vals[5];
+        } else {
+            mesh[idx] = vals[0];
+            mesh[nmesh + idx] = vals[1];
+            mesh[2 * nmesh + idx] = vals[2];
+            mesh[3 * nmesh + 3 * idx + 0] = vals[3];
+            mesh[3 * nmesh + 3 * idx + 1] = vals[4];
+            mesh[3 * nmesh + 3 * idx + 2] = vals[5];
+        }
+    }
+}
+
 // One workgroup per tile: FP64 LDS accumulation, then one flush of the tile.
 __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
     DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
@@ -308,33 +350,143 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
         }
     }
     __syncthreads();
+    flush_tile(a, acc, cx0, cy0, cz0, mesh, accumulate);
+}
 
-    const int64_t nc = a.nc;
-    const int64_t nmesh = nc * nc * nc;
-    for (int c = tid; c < ncell; c += kDepThreads) {
-        const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
-        const int gx = cx0 + lx, gy = cy0 + ly, gz = cz0 + lz;
-        if (gx >= a.nc || gy >= a.nc || gz >= a.nc) continue;
-        const int64_t idx = (static_cast<int64_t>(gz) * nc + gy) * nc + gx;
-        float vals[6];
+// Node mesh, one workgroup per tile, with the (particle, cell) terms packed
+// onto lanes.  A wavefront takes 64 list entries at a time (entries `nb`
+// apart, so that they lie far apart in space and their cells rarely
+// coincide): lane l computes entry l's box of tile cells and its per-axis
+// weights (into LDS), an exclusive scan of the box sizes lays the boxes end
+// to end, and the wavefront sweeps that sequence 64 cells at a time.  Lane l
+// takes cell base + l: the entries whose boxes start inside the sweep mark
+// their start in a per-wavefront LDS map (tagged with the sweep, so it never
+// needs clearing), a ballot of the marks gives every lane its entry, and the
+// lane adds the five or one terms of its cell with ds_add_f64.  Lanes of one
+// entry cover consecutive x cells (consecutive LDS words), every lane does
+// useful work whatever the box shapes, and each term is the reference's
+// FP32 product ((wx * wy) * wz, coarse_mesh.cpp:93).
+constexpr int kDepWaves = kDepThreads / kWave;
+constexpr int kWStride = 3 * kTileMax + 1;   // odd row stride: entries fall in different banks
+
+__global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
+    DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
+    const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
+{
+    __shared__ double acc[6][kTileCells];
+    __shared__ float wts[kDepWaves][kWave * kWStride];
+    __shared__ int smap[kDepWaves][kWave];
+    const int tid = threadIdx.x;
+    const int lane = tid % kWave;
+    const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
+    float* const W = wts[wid];
+    int* const map = smap[wid];
+    const int T = a.tile;
+    const int64_t tile = blockIdx.x;
+    const int tx = static_cast<int>(tile % a.ntx);
+    const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
+    const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
+    const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
+    for (int c = tid; c < 6 * kTileCells; c += kDepThreads) (&acc[0][0])[c] = 0.0;
+    map[lane] = -1;
+    __syncthreads();
+
+    const float dxc = a.boxsize / static_cast<float>(a.nc);
+    const uint32_t begin = tile_start[tile];
+    const uint32_t count = tile_cnt[tile];
+    const uint32_t nb = (count + kWave - 1) / kWave;   // batches; batch b holds entries l * nb + b
+    int tag = 0;                                       // sweep counter, tags the map marks
+    for (uint32_t b = wid; b < nb; b += kDepWaves) {
+        const uint32_t e = static_cast<uint32_t>(lane) * nb + b;
+        // Entry setup: box of tile cells [b0, b0 + n) per axis and weights.
+        int size = 0;
+        uint32_t packed = 0;    // b0x | b0y << 4 | b0z << 8 | nx << 12 | ny << 16 | igm << 20
+        float dens = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
+        if (e < count) {
+            const int64_t i = list[begin + e];
+            const PartRange r = part_range(a, i);
+            const float hinv = 1.0f / r.h;
+            const float ds = dxc * hinv;
+            const float4 at = a.sattr[i];
+            dens = a.dens[i];
+            vx = at.x;
+            vy = at.y;
+            vz = at.z;
+            const int igm = (a.sign * __float_as_int(at.w) <= 0) ? 1 : 0;
+            const int c0[3] = {cx0, cy0, cz0};
+            int b0[3], n3[3];
 #pragma unroll
-        for (int f = 0; f < 6; ++f) vals[f] = static_cast<float>(acc[f][c]);
-        if (accumulate) {
-            mesh[idx] += vals[0];
-            mesh[nmesh + idx] += vals[1];
-            mesh[2 * nmesh + idx] += vals[2];
-            mesh[3 * nmesh + 3 * idx + 0] += vals[3];
-            mesh[3 * nmesh + 3 * idx + 1] += vals[4];
-            mesh[3 * nmesh + 3 * idx + 2] += vals[5];
-        } else {
-            mesh[idx] = vals[0];
-            mesh[nmesh + idx] = vals[1];
-            mesh[2 * nmesh + idx] = vals[2];
-            mesh[3 * nmesh + 3 * idx + 0] = vals[3];
-            mesh[3 * nmesh + 3 * idx + 1] = vals[4];
-            mesh[3 * nmesh + 3 * idx + 2] = vals[5];
+            for (int d = 0; d < 3; ++d) {
+                const int lo = max(r.lo[d], c0[d]);
+                const int hi = min(r.hi[d], c0[d] + T);
+                b0[d] = lo - c0[d];
+                n3[d] = hi > lo ? hi - lo : 0;
+                for (int t = 0; t < n3[d]; ++t) {
+                    const float s1 = (static_cast<float>(lo + t) * dxc - r.x[d]) * hinv;
+                    W[lane * kWStride + d * kTileMax + t] = ker_integ(s1, s1 + ds);
+                }
+            }
+            size = n3[0] * n3[1] * n3[2];
+            packed = static_cast<uint32_t>(b0[0] | (b0[1] << 4) | (b0[2] << 8) | (n3[0] << 12) | (n3[1] << 16) |
+                                           (igm << 20));
         }
+        // Exclusive scan of the box sizes.
+        int incl = size;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const int pre = incl - size;
+        const int total = __shfl(incl, kWave - 1);
+        int first = -1;  // entry holding the cell just before the sweep
+        for (int base = 0; base < total; base += kWave, ++tag) {
+            if (size > 0 && pre >= base && pre < base + kWave) map[pre - base] = tag;
+            wave_lds_fence_dep();
+            const uint64_t sb = __ballot(map[lane] == tag);   // cells where an entry starts
+            const int pos = base + lane;
+            // Entry of this cell: the entry before the sweep plus the number
+            // of starts at or below this lane.
+            const int p = first + __popcll(sb & ((2ull << lane) - 1ull));
+            first = __shfl(p, kWave - 1);
+            const int p_pre = __shfl(pre, p);
+            const uint32_t pk = static_cast<uint32_t>(__shfl(static_cast<int>(packed), p));
+            const float p_dens = __shfl(dens, p);
+            const float p_vx = __shfl(vx, p);
+            const float p_vy = __shfl(vy, p);
+            const float p_vz = __shfl(vz, p);
+            if (pos < total) {
+                const int bx = pk & 15, by = (pk >> 4) & 15, bz = (pk >> 8) & 15;
+                const int nx = (pk >> 12) & 15, ny = (pk >> 16) & 15;
+                const bool igm = (pk >> 20) & 1;
+                const int off = pos - p_pre;
+                const int nxy = nx * ny;
+                const int iz = static_cast<int>((static_cast<float>(off) + 0.5f) *
+                                                __builtin_amdgcn_rcpf(static_cast<float>(nxy)));
+                const int rem = off - iz * nxy;
+                const int iy = static_cast<int>((static_cast<float>(rem) + 0.5f) *
+                                                __builtin_amdgcn_rcpf(static_cast<float>(nx)));
+                const int ix = rem - iy * nx;
+                const float* const wp = W + p * kWStride;
+                // coarse_mesh.cpp:93: ker3 = (wx * wy) * wz
+                float ker3 = wp[ix] * wp[kTileMax + iy];
+                ker3 = ker3 * wp[2 * kTileMax + iz];
+                const int c = ((bz + iz) * T + (by + iy)) * T + bx + ix;
+                if (igm) {
+                    atomicAdd(&acc[0][c], static_cast<double>(ker3));
+                    atomicAdd(&acc[2][c], static_cast<double>(ker3 * p_dens));
+                    atomicAdd(&acc[3][c], static_cast<double>(ker3 * p_vx));
+                    atomicAdd(&acc[4][c], static_cast<double>(ker3 * p_vy));
+                    atomicAdd(&acc[5][c], static_cast<double>(ker3 * p_vz));
+                } else {
+                    atomicAdd(&acc[1][c], static_cast<double>(ker3));
+                }
+            }
+        }
+        wave_lds_fence_dep();
     }
+    __syncthreads();
+    flush_tile(a, acc, cx0, cy0, cz0, mesh, accumulate);
 }
 
 template <typename T>
@@ -443,8 +595,12 @@ hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start
 {
     if (a.tile < 1 || a.tile > kTileMax) return hipErrorInvalidValue;
     const int64_t ntiles = static_cast<int64_t>(a.ntx) * a.ntx * a.ntx;
-    deposit_tile_kernel<<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(a, tile_start, tile_cnt, list,
-                                                                              mesh, accumulate);
+    if (a.periodic)
+        deposit_tile_kernel<<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(a, tile_start, tile_cnt, list,
+                                                                                  mesh, accumulate);
+    else
+        deposit_tile_packed_kernel<<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(a, tile_start, tile_cnt,
+                                                                                         list, mesh, accumulate);
     return hipGetLastError();
 }
 
