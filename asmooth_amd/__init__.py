@@ -109,6 +109,7 @@ def lib() -> C.CDLL:
         "asm_get_timer": (i32, [vp, i32, P(f64), P(i64)]),
         "asm_reset_timers": (i32, [vp]),
         "asm_get_search_stats": (i32, [vp, P(C.c_uint64)]),
+        "asm_autotune": (i32, [vp, P(i32), i32, P(i32), i32, i32, P(i32), P(i32)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -282,6 +283,18 @@ class Context:
         out = (C.c_uint64 * 4)()
         self._check(self._L.asm_get_search_stats(self._h, out), "asm_get_search_stats")
         return dict(zip(["node_tests", "leaf_scans", "dist_evals", "inserts"], list(out)))
+
+    def autotune(self, leaf_sizes=(16, 24, 32), ncs=(), sign: int = -1) -> tuple[int, dict[int, int]]:
+        """Time leaf sizes and per-nc deposition tiles on this context's
+        particles and keep the fastest (asm_autotune).  Returns (leaf size,
+        {nc: tile})."""
+        ls = (C.c_int32 * max(len(leaf_sizes), 1))(*leaf_sizes)
+        nn = (C.c_int32 * max(len(ncs), 1))(*ncs)
+        tiles = (C.c_int32 * max(len(ncs), 1))()
+        leaf = C.c_int32()
+        self._check(self._L.asm_autotune(self._h, ls, len(leaf_sizes), nn, len(ncs), int(sign), C.byref(leaf), tiles),
+                    "asm_autotune")
+        return leaf.value, {int(nc): int(tiles[i]) for i, nc in enumerate(ncs)}
 
 
 # -- host steps either side of the path ------------------------------------

@@ -70,6 +70,8 @@ struct asm_context {
     int profile = 0;
     int leaf_size = 32;
     int tile = 8;
+    std::map<int, int> tile_for_nc;   // set by asm_autotune
+    float tree_left = 0.0f, tree_right = 0.0f;
 
     // particle state
     int64_t n = 0;
@@ -278,6 +280,7 @@ int asm_set_option(asm_context* ctx, int option, int value)
     case ASM_OPT_TILE:
         if (value < 1 || value > 8) return fail(ctx, ASM_ERR_ARG, "tile %d outside 1..8", value);
         ctx->tile = value;
+        ctx->tile_for_nc.clear();
         return ASM_OK;
     default: return fail(ctx, ASM_ERR_ARG, "unknown option %d", option);
     }
@@ -325,6 +328,8 @@ int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode
     if (!ctx->have_particles) return fail(ctx, ASM_ERR_STATE, "asm_tree_construct before asm_set_particles");
     if (!(right > left)) return fail(ctx, ASM_ERR_ARG, "tree cube [%g, %g] is empty", left, right);
     HIPC(hipSetDevice(ctx->device));
+    ctx->tree_left = left;
+    ctx->tree_right = right;
     const int64_t n = ctx->n;
     hipStream_t s = ctx->stream;
     HIPC(ctx->keys.ensure(n));
@@ -462,8 +467,11 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     a.boxsize = ctx->boxsize;
     a.nc = nc;
     a.sign = sign;
-    a.tile = ctx->tile;
-    a.ntx = (nc + ctx->tile - 1) / ctx->tile;
+    {
+        const auto it = ctx->tile_for_nc.find(nc);
+        a.tile = (it != ctx->tile_for_nc.end() && !periodic) ? it->second : ctx->tile;
+    }
+    a.ntx = (nc + a.tile - 1) / a.tile;
     a.periodic = periodic;
     a.coord[0] = coord ? coord[0] : 0;
     a.coord[1] = coord ? coord[1] : 0;
@@ -521,6 +529,87 @@ int asm_assign_on_global_mesh(asm_context* ctx, int nc, int sign, const int coor
 {
     if (!ctx || !coord) return fail(ctx, ASM_ERR_ARG, "null argument");
     return deposit_one(ctx, nc, sign, 1, coord);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Milliseconds of fn() on the context stream (one warm-up call, then one timed).
+template <typename F>
+int time_on_stream(asm_context* ctx, F fn, float* ms)
+{
+    int rc = fn();
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, ctx->stream));
+    rc = fn();
+    HIPC(hipEventRecord(e1, ctx->stream));
+    HIPC(hipEventSynchronize(e1));
+    HIPC(hipEventElapsedTime(ms, e0, e1));
+    HIPC(hipEventDestroy(e0));
+    HIPC(hipEventDestroy(e1));
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int asm_autotune(asm_context* ctx, const int* leaf_sizes, int n_leaf, const int* ncs, int n_nc, int sign,
+                 int* leaf_out, int* tiles_out)
+{
+    if (!ctx || (n_leaf > 0 && !leaf_sizes) || (n_nc > 0 && !ncs)) return fail(ctx, ASM_ERR_ARG, "null argument");
+    if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "asm_autotune before asm_neighbor_search");
+    HIPC(hipSetDevice(ctx->device));
+    const int profile = ctx->profile;
+    ctx->profile = 0;   // keep the stage timers for the caller's own runs
+    int rc = ASM_OK;
+    if (n_leaf > 0) {
+        int best = ctx->leaf_size;
+        float best_ms = INFINITY;
+        for (int i = 0; i < n_leaf && rc == ASM_OK; ++i) {
+            const int ls = leaf_sizes[i];
+            if (ls < 8 || ls > kWave) {
+                rc = fail(ctx, ASM_ERR_ARG, "leaf size %d outside 8..64", ls);
+                break;
+            }
+            ctx->leaf_size = ls;
+            float ms = 0.0f;
+            rc = time_on_stream(ctx, [&]() {
+                int r = asm_tree_construct(ctx, ctx->tree_left, ctx->tree_right, nullptr);
+                return r ? r : asm_neighbor_search(ctx, kSlots - 1);
+            }, &ms);
+            if (rc == ASM_OK && ms < best_ms) {
+                best_ms = ms;
+                best = ls;
+            }
+        }
+        ctx->leaf_size = best;
+        if (rc == ASM_OK) rc = asm_tree_construct(ctx, ctx->tree_left, ctx->tree_right, nullptr);
+        if (rc == ASM_OK) rc = asm_neighbor_search(ctx, kSlots - 1);
+    }
+    if (leaf_out) *leaf_out = ctx->leaf_size;
+    for (int i = 0; i < n_nc && rc == ASM_OK; ++i) {
+        const int nc = ncs[i];
+        int best = ctx->tile;
+        float best_ms = INFINITY;
+        for (int tile = 4; tile <= 8 && rc == ASM_OK; ++tile) {
+            ctx->tile_for_nc[nc] = tile;
+            float ms = 0.0f;
+            rc = time_on_stream(ctx, [&]() { return deposit_one(ctx, nc, sign, 0, nullptr); }, &ms);
+            if (rc == ASM_OK && ms < best_ms) {
+                best_ms = ms;
+                best = tile;
+            }
+        }
+        ctx->tile_for_nc[nc] = best;
+        if (tiles_out) tiles_out[i] = best;
+    }
+    ctx->profile = profile;
+    return rc;
 }
 
 int asm_mesh_stats(asm_context* ctx, int nc, double out[4])

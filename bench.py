@@ -104,6 +104,10 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-side", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--leaf-size", type=int, default=32, help="particles per tree leaf (8..64)")
+    ap.add_argument("--tile", type=int, default=8, help="deposition tile side in cells (1..8)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --leaf-size/--tile instead of timing the candidates before the timed steps")
     ap.add_argument("--knn-method", type=int, default=0,
                     help="0: bound walk + collect walk (default); 1: sorted-key insertion lists")
     args = ap.parse_args(argv)
@@ -126,10 +130,21 @@ def main(argv=None) -> int:
     ctx.set_particles(parts, nl, L)
     ctx.set_option(A.OPT_PROFILE, 1)
     ctx.set_option(A.OPT_KNN_METHOD, args.knn_method)
+    ctx.set_option(A.OPT_LEAF_SIZE, args.leaf_size)
+    ctx.set_option(A.OPT_TILE, args.tile)
     for i in range(args.warmup):
         run_step(ctx, L, bf, ncs)
         ctx.synchronize()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    tuned = None
+    if not args.no_autotune:
+        # One-time choice of leaf size and per-mesh tile on this particle set
+        # (not part of a step; the timed steps then run with the choice).
+        t = time.time()
+        leaf, tiles = ctx.autotune((16, 24, 32), ncs, -1)
+        tuned = {"leaf_size": leaf, "tile_per_nc": {str(k): v for k, v in tiles.items()},
+                 "seconds": round(time.time() - t, 2)}
+        log(f"[bench] autotune: leaf {leaf}, tiles {tiles} ({tuned['seconds']} s)")
     ctx.reset_timers()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -170,7 +185,9 @@ def main(argv=None) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"zeldovich {n_side}^3 L={L:g} bf={bf} k=32 nc={','.join(map(str, ncs))}",
-                   "n_local": nl, "n_buf": nbuf, "ncs": ncs, "leaf_size": 32, "tile": 8, "knn_method": args.knn_method,
+                   "n_local": nl, "n_buf": nbuf, "ncs": ncs, "leaf_size": tuned["leaf_size"] if tuned else args.leaf_size,
+                   "tile": tuned["tile_per_nc"] if tuned else args.tile, "autotuned": tuned is not None,
+                   "knn_method": args.knn_method,
                    "path_roofline_frac": round(b_alg / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 6),
                    "mass_check_ok": bool(mass_ok),
                    "stage_ms": {k: round(v[0] / args.steps, 4) for k, v in timers.items() if v[1]},

@@ -186,6 +186,18 @@ void asm_free(void* p);
  * floats) and vel (int32 nc + 3 nc^3 floats). */
 int asm_write_mesh_separate(const char* base, int inode, const float* mesh, int nc);
 
+/* Autotuner (no counterpart in the reference; the reference has fixed
+ * choices: leaf quota 16, kdtree_balanced.cpp:245, and no tiling).  After a
+ * search, times the tree build + search for each leaf size in leaf_sizes
+ * (n_leaf may be 0 to keep the current one) and the deposition of each nc in
+ * ncs for every tile side 4..8, on the context's own particles, and keeps the
+ * fastest: later trees use *leaf_out and later depositions of nc use
+ * tiles_out[i] (until ASM_OPT_TILE or ASM_OPT_LEAF_SIZE is set).  The tree
+ * and search are redone with the chosen leaf size, so the context is ready
+ * for asm_assign_on_mesh.  Results do not depend on either choice. */
+int asm_autotune(asm_context* ctx, const int* leaf_sizes, int n_leaf, const int* ncs, int n_nc, int sign,
+                 int* leaf_out, int* tiles_out);
+
 /* Timers (HIP events on the context stream) when ASM_OPT_PROFILE is set. */
 int asm_timer_count(void);
 const char* asm_timer_name(int i);

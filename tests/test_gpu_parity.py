@@ -183,6 +183,31 @@ def test_knn_methods_agree(ctx, kind, n_side, seed):
     assert rel_err(r0[1], r1[1]) <= DENS_RTOL
 
 
+def test_autotune_keeps_results(ctx):
+    xv, L, bf = small_set("zeldovich", 16, 12)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    ncs = [9, 16]
+    o = oracle_path(buf, L, ncs)
+    ctx.set_option(A.OPT_KEEP_NEIGHBORS, 0)
+    ctx.set_particles(parts, nl, L)
+    ctx.construct(-bf * L, (1 + bf) * L)
+    ctx.neighbor_search(32)
+    try:
+        leaf, tiles = ctx.autotune((16, 24, 32), ncs, -1)
+        assert leaf in (16, 24, 32) and set(tiles) == set(ncs) and all(4 <= t <= 8 for t in tiles.values())
+        rk, dens, _ = ctx.get_rk_dens()
+        assert np.array_equal(rk.view(np.int32), o["rk"].view(np.int32))
+        assert rel_err(dens, o["dens64"]) <= DENS_RTOL
+        vmax = float(np.max(np.abs(buf[:, 3:])))
+        for nc in ncs:
+            ctx.assign_on_mesh(nc, -1)
+            _check_mesh(ctx.get_mesh(nc), o["meshes"][nc], nc, vmax)
+    finally:
+        ctx.set_option(A.OPT_LEAF_SIZE, 32)
+        ctx.set_option(A.OPT_TILE, 8)
+
+
 def test_too_few_particles_is_an_error(ctx):
     parts = A.particles_from_xv(np.zeros((10, 6), np.float32))
     with pytest.raises(A.AsmoothError):
