@@ -27,7 +27,7 @@ import numpy as np
 __all__ = [
     "PARTICLE_DTYPE", "HALO_DTYPE", "K_SLOTS", "AsmoothError", "Context", "lib",
     "build_buffered_set", "build_node_share", "build_buffered_halos", "read_halo_file", "halo_delta_vir", "read_xv", "write_xv", "write_mesh_separate",
-    "particles_from_xv", "timer_names",
+    "particles_from_xv", "timer_names", "Mlp", "mlp_init_params",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -110,6 +110,18 @@ def lib() -> C.CDLL:
         "asm_reset_timers": (i32, [vp]),
         "asm_get_search_stats": (i32, [vp, P(C.c_uint64)]),
         "asm_autotune": (i32, [vp, P(i32), i32, P(i32), i32, i32, P(i32), P(i32)]),
+        "asm_mlp_param_count": (i32, []),
+        "asm_mlp_create": (i32, [P(vp), i32]),
+        "asm_mlp_destroy": (None, [vp]),
+        "asm_mlp_last_error": (C.c_char_p, [vp]),
+        "asm_mlp_set_params": (i32, [vp, vp]),
+        "asm_mlp_get_params": (i32, [vp, vp]),
+        "asm_mlp_forward": (i32, [vp, vp, i64, vp]),
+        "asm_mlp_loss_grad": (i32, [vp, vp, vp, i64, P(f64), vp]),
+        "asm_mlp_train": (i32, [vp, vp, vp, i64, i32, i32, C.c_float, C.c_uint64, vp]),
+        "asm_seed_samples": (i32, [vp, vp, i64, vp, vp]),
+        "asm_set_seed_model": (i32, [vp, vp, C.c_float]),
+        "asm_get_seed_stats": (i32, [vp, P(i64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -284,6 +296,24 @@ class Context:
         self._check(self._L.asm_get_search_stats(self._h, out), "asm_get_search_stats")
         return dict(zip(["node_tests", "leaf_scans", "dist_evals", "inserts"], list(out)))
 
+    # -- learned seed radius -----------------------------------------------
+    def seed_samples(self, idx: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """Features (m, 27) and targets ln(h / spacing) of particles idx (caller order)."""
+        idx = np.ascontiguousarray(idx, np.int64)
+        x = np.empty((len(idx), MLP_INPUTS), np.float32)
+        t = np.empty(len(idx), np.float32)
+        self._check(self._L.asm_seed_samples(self._h, _ptr(idx), len(idx), _ptr(x), _ptr(t)), "asm_seed_samples")
+        return x, t
+
+    def set_seed_model(self, model: "Mlp | None", margin: float = 0.2) -> None:
+        self._check(self._L.asm_set_seed_model(self._h, model._h if model is not None else None, float(margin)),
+                    "asm_set_seed_model")
+
+    def seed_stats(self) -> dict[str, int]:
+        out = (C.c_int64 * 3)()
+        self._check(self._L.asm_get_seed_stats(self._h, out), "asm_get_seed_stats")
+        return {"fallback_particles": out[0], "fallback_wavefronts": out[1], "particles": out[2]}
+
     def autotune(self, leaf_sizes=(16, 24, 32), ncs=(), sign: int = -1) -> tuple[int, dict[int, int]]:
         """Time leaf sizes and per-nc deposition tiles on this context's
         particles and keep the fastest (asm_autotune).  Returns (leaf size,
@@ -295,6 +325,88 @@ class Context:
         self._check(self._L.asm_autotune(self._h, ls, len(leaf_sizes), nn, len(ncs), int(sign), C.byref(leaf), tiles),
                     "asm_autotune")
         return leaf.value, {int(nc): int(tiles[i]) for i, nc in enumerate(ncs)}
+
+
+MLP_INPUTS = 27
+MLP_HIDDEN = 64
+
+
+class Mlp:
+    """The learned seed radius model (asm_mlp): a 27-64-64-1 GELU network,
+    trained and evaluated on the device."""
+
+    def __init__(self, device: int = 0, params: np.ndarray | None = None):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.asm_mlp_create(C.byref(h), int(device))
+        if rc != 0:
+            raise AsmoothError(f"asm_mlp_create(device={device}) failed with status {rc}")
+        self._h = h
+        self.n_params = int(self._L.asm_mlp_param_count())
+        if params is not None:
+            self.set_params(params)
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc < 0:
+            msg = self._L.asm_mlp_last_error(self._h).decode(errors="replace")
+            raise AsmoothError(f"{what}: status {rc}: {msg}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.asm_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.n_params:
+            raise AsmoothError(f"expected {self.n_params} parameters, got {p.size}")
+        self._check(self._L.asm_mlp_set_params(self._h, _ptr(p)), "asm_mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        self._check(self._L.asm_mlp_get_params(self._h, _ptr(p)), "asm_mlp_get_params")
+        return p
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        y = np.empty(len(x), np.float32)
+        self._check(self._L.asm_mlp_forward(self._h, _ptr(x), len(x), _ptr(y)), "asm_mlp_forward")
+        return y
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_double()
+        self._check(self._L.asm_mlp_loss_grad(self._h, _ptr(x), _ptr(t), len(x), C.byref(loss), _ptr(g)),
+                    "asm_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, x: np.ndarray, t: np.ndarray, epochs: int = 5, batch: int = 4096, lr: float = 1e-3,
+              seed: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32)
+        losses = np.empty(epochs, np.float32)
+        self._check(self._L.asm_mlp_train(self._h, _ptr(x), _ptr(t), len(x), int(epochs), int(batch), float(lr),
+                                          int(seed), _ptr(losses)), "asm_mlp_train")
+        return losses
+
+
+def mlp_init_params(seed: int = 0) -> np.ndarray:
+    """He-style random initialisation in the asm_mlp parameter layout
+    (W1[27][64], b1, W2[64][64], b2, w3[64], b3)."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.standard_normal((MLP_INPUTS, MLP_HIDDEN)) * np.sqrt(2.0 / MLP_INPUTS)
+    w2 = rng.standard_normal((MLP_HIDDEN, MLP_HIDDEN)) * np.sqrt(2.0 / MLP_HIDDEN)
+    w3 = rng.standard_normal(MLP_HIDDEN) * np.sqrt(1.0 / MLP_HIDDEN)
+    return np.concatenate([w1.ravel(), np.zeros(MLP_HIDDEN), w2.ravel(), np.zeros(MLP_HIDDEN), w3,
+                           np.zeros(1)]).astype(np.float32)
 
 
 # -- host steps either side of the path ------------------------------------

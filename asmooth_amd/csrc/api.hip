@@ -51,10 +51,11 @@ enum TimerId {
     T_REDUCE,
     T_NORMALIZE,
     T_HALO,
+    T_SEED,
     T_COUNT
 };
 const char* kTimerNames[T_COUNT] = {"sort", "tree", "knn", "dens", "deposit_lists", "deposit_tiles",
-                                    "mesh_reduce", "normalize", "mark_halo"};
+                                    "mesh_reduce", "normalize", "mark_halo", "seed"};
 
 struct MeshDev {
     DevBuf<float> mesh;
@@ -98,6 +99,13 @@ struct asm_context {
     DevBuf<uint32_t> knn_flags, knn_leaf_n, knn_leaves;   // per wavefront of 64 sorted particles
     DevBuf<float> knn_tau;                                // per sorted particle
     DevBuf<double> dens_pull;
+    // learned seed radius
+    bool seed_on = false;
+    float seed_margin = 0.0f;
+    DevBuf<float> seed_params, seed_grid, tau_init;
+    DevBuf<uint32_t> seed_redo;
+    DevBuf<unsigned long long> seed_fb;
+    bool seed_used = false;   // last search used the model
     DevBuf<int32_t> nbr;
     DevBuf<KnnStats> kstats;   // kStatSlots copies
     // deposition
@@ -384,7 +392,26 @@ int asm_neighbor_search(asm_context* ctx, int k)
         HIPC(ctx->knn_tau.ensure(n));
         if (!ctx->deterministic) HIPC(ctx->dens_pull.ensure(n));
     }
-    const KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p};
+    KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p,
+                    nullptr, nullptr, nullptr};
+    ctx->seed_used = false;
+    if (bound_collect && ctx->seed_on) {
+        HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));
+        HIPC(ctx->tau_init.ensure(n));
+        HIPC(ctx->seed_redo.ensure(nwaves));
+        HIPC(ctx->seed_fb.ensure(kStatSlots));
+        const float side = ctx->tree_right - ctx->tree_left;
+        const float spacing = std::cbrt(side * side * side / static_cast<float>(n));
+        StageTimer t(ctx, T_SEED);
+        HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->seed_fb.p), 2 * kStatSlots, s));
+        HIPC(launch_seed_grid(ctx->spos.p, n, ctx->tree_left, ctx->tree_right, ctx->seed_grid.p, s));
+        HIPC(launch_seed_bound(ctx->spos.p, n, ctx->seed_grid.p, ctx->tree_left, ctx->tree_right, ctx->seed_params.p,
+                               spacing * (1.0f + ctx->seed_margin), ctx->tau_init.p, s));
+        kscr.tau_init = ctx->tau_init.p;
+        kscr.seed_redo = ctx->seed_redo.p;
+        kscr.seed_fallbacks = ctx->seed_fb.p;
+        ctx->seed_used = true;
+    }
     int32_t* nbr = nullptr;
     if (ctx->keep_neighbors) {
         HIPC(ctx->nbr.ensure(static_cast<size_t>(n) * kSlots));
@@ -610,6 +637,76 @@ int asm_autotune(asm_context* ctx, const int* leaf_sizes, int n_leaf, const int*
     }
     ctx->profile = profile;
     return rc;
+}
+
+int asm_set_seed_model(asm_context* ctx, const asm_mlp* m, float margin)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    if (!m) {
+        ctx->seed_on = false;
+        return ASM_OK;
+    }
+    if (!(margin >= 0.0f)) return fail(ctx, ASM_ERR_ARG, "seed margin %g is negative", margin);
+    HIPC(hipSetDevice(ctx->device));
+    std::vector<float> p(kMlpParams);
+    const int rc = asm_mlp_get_params(const_cast<asm_mlp*>(m), p.data());
+    if (rc) return fail(ctx, rc, "cannot read the seed model's parameters");
+    HIPC(ctx->seed_params.ensure(kMlpParams));
+    HIPC(hipMemcpyAsync(ctx->seed_params.p, p.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    ctx->seed_on = true;
+    ctx->seed_margin = margin;
+    return ASM_OK;
+}
+
+int asm_get_seed_stats(asm_context* ctx, int64_t out[3])
+{
+    if (!ctx || !out) return ASM_ERR_ARG;
+    out[0] = out[1] = 0;
+    out[2] = ctx->n;
+    if (!ctx->seed_used) return ASM_OK;
+    HIPC(hipSetDevice(ctx->device));
+    const size_t nwaves = static_cast<size_t>((ctx->n + kWave - 1) / kWave);
+    std::vector<unsigned long long> fb(kStatSlots);
+    std::vector<uint32_t> redo(nwaves);
+    HIPC(hipMemcpyAsync(fb.data(), ctx->seed_fb.p, sizeof(unsigned long long) * kStatSlots, hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipMemcpyAsync(redo.data(), ctx->seed_redo.p, sizeof(uint32_t) * nwaves, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    for (unsigned long long v : fb) out[0] += static_cast<int64_t>(v);
+    for (uint32_t v : redo) out[1] += v ? 1 : 0;
+    return ASM_OK;
+}
+
+int asm_seed_samples(asm_context* ctx, const int64_t* idx, int64_t m, float* x, float* target)
+{
+    if (!ctx || m < 0 || (m > 0 && (!idx || !x || !target))) return fail(ctx, ASM_ERR_ARG, "null argument");
+    if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "asm_seed_samples before asm_neighbor_search");
+    if (m == 0) return ASM_OK;
+    for (int64_t b = 0; b < m; ++b)
+        if (idx[b] < 0 || idx[b] >= ctx->n) return fail(ctx, ASM_ERR_ARG, "index %lld outside the particles",
+                                                       static_cast<long long>(idx[b]));
+    HIPC(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));
+    DevBuf<int64_t> didx;
+    DevBuf<int32_t> sidx;
+    DevBuf<float> dx, dt;
+    HIPC(didx.ensure(m));
+    HIPC(sidx.ensure(m));
+    HIPC(dx.ensure(static_cast<size_t>(m) * kMlpIn));
+    HIPC(dt.ensure(m));
+    const float side = ctx->tree_right - ctx->tree_left;
+    const float spacing = std::cbrt(side * side * side / static_cast<float>(ctx->n));
+    HIPC(hipMemcpyAsync(didx.p, idx, sizeof(int64_t) * m, hipMemcpyHostToDevice, s));
+    HIPC(launch_seed_grid(ctx->spos.p, ctx->n, ctx->tree_left, ctx->tree_right, ctx->seed_grid.p, s));
+    HIPC(launch_seed_samples(ctx->spos.p, ctx->rk.p, ctx->inv_perm.p, didx.p, m, ctx->seed_grid.p, ctx->tree_left,
+                             ctx->tree_right, 1.0f / spacing, sidx.p, dx.p, dt.p, s));
+    HIPC(hipMemcpyAsync(x, dx.p, sizeof(float) * m * kMlpIn, hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(target, dt.p, sizeof(float) * m, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return ASM_OK;
 }
 
 int asm_mesh_stats(asm_context* ctx, int nc, double out[4])

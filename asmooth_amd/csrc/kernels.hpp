@@ -64,6 +64,13 @@ struct KnnScratch {
     uint32_t* wave_flags;
     float* tau;
     double* dens_pull;
+    // Learned seed (all null without it): per sorted particle an initial
+    // bound on the 33rd squared distance; per wavefront a flag set when a
+    // lane's bound held fewer than 33 particles; fallback lanes counted in
+    // seed_fallbacks[wave % kStatSlots].
+    const float* tau_init;
+    uint32_t* seed_redo;
+    unsigned long long* seed_fallbacks;
 };
 hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
                       int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, const int32_t* inv_perm, double* dens64,
@@ -76,6 +83,28 @@ hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t n
 // Density by gather (sorted order, float).
 hipError_t launch_dens_gather(const float4* spos, const uint64_t* kkey, const float* rk, int64_t n, const Box* nodes,
                               const float* kmax, int64_t nleaf_pad, float* dens, hipStream_t s);
+
+// ---- mlp.hip -----------------------------------------------------------------
+// Learned seed radius: 27 inputs (ln CIC density over 3x3x3 cells of a
+// kMlpGrid^3 mesh of the tree cube), hidden layers 64-64 (GELU), output
+// ln(h / spacing).
+constexpr int kMlpIn = 27;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden + kMlpHidden + 1;
+constexpr int kMlpGrid = 64;
+constexpr float kMlpLnFloor = 1e-3f;   // ln(density / mean + floor)
+// grid (kMlpGrid^3 floats) <- ln(CIC density / mean + floor) over [left, right)^3.
+hipError_t launch_seed_grid(const float4* spos, int64_t n, float left, float right, float* grid, hipStream_t s);
+// x[b] (kMlpIn floats) = features of the particle at sorted position sidx[b].
+hipError_t launch_seed_features(const float4* spos, const int32_t* sidx, int64_t m, const float* lngrid, float left,
+                                float right, float* x, hipStream_t s);
+// Training samples of caller indices idx[0..m): features and ln(h / spacing).
+hipError_t launch_seed_samples(const float4* spos, const float* rk_sorted, const int32_t* inv_perm,
+                               const int64_t* idx, int64_t m, const float* lngrid, float left, float right,
+                               float inv_spacing, int32_t* sidx, float* x, float* target, hipStream_t s);
+// tau_init[q] = (exp(f(x_q)) * scale)^2, scale = spacing * (1 + margin).
+hipError_t launch_seed_bound(const float4* spos, int64_t n, const float* lngrid, float left, float right,
+                             const float* params, float scale, float* tau_init, hipStream_t s);
 
 // ---- deposit.hip -------------------------------------------------------------
 struct DepositArgs {

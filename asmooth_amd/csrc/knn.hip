@@ -319,11 +319,19 @@ __device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int6
 // the leaves opened here include every leaf that walk 2 needs; their ids are
 // recorded per wavefront (up to kLeafCap; beyond that the count alone is
 // kept and walk 2 hands the wavefront to the insertion kernel).
+//
+// With a learned bound tau_init (per sorted particle), t starts as 33 copies
+// of it: real distances below it displace the copies, so t[K-1] < tau_init at
+// the end means the 33 smallest were all found below the bound and the
+// result is exact; otherwise the lane's wavefront is flagged in seed_redo
+// and searched again with only_flagged = seed_redo and no bound.
 template <int K>
 __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
     int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
-    uint32_t* __restrict__ wleaf_n, KnnStats* __restrict__ stats)
+    uint32_t* __restrict__ wleaf_n, KnnStats* __restrict__ stats, const float* __restrict__ tau_init,
+    const uint32_t* __restrict__ only_flagged, uint32_t* __restrict__ seed_redo,
+    unsigned long long* __restrict__ seed_fallbacks)
 {
     __shared__ float4 stage[kBoundWaves][kWave];
     const int lane = threadIdx.x % kWave;
@@ -331,14 +339,16 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kBoundWaves + wid) * kWave;
     if (wave0 >= n) return;
     const int64_t wave = wave0 / kWave;
+    if (only_flagged && only_flagged[wave] == 0u) return;
     const int64_t q = wave0 + lane;
     const bool valid = q < n;
     const float4 me = spos[valid ? q : n - 1];
     const float qx = me.x, qy = me.y, qz = me.z;
+    const float t0 = (tau_init && valid) ? tau_init[q] : INFINITY;
 
     float t[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) t[i] = INFINITY;
+    for (int i = 0; i < K; ++i) t[i] = t0;
 
     unsigned long long n_node = 0, n_leaf = 0, n_dist = 0;
     uint32_t n_acc = 0;
@@ -415,6 +425,13 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
         my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
     if (lane == 0) wleaf_n[wave] = static_cast<uint32_t>(n_rec);
     if (valid) tau_out[q] = t[K - 1];
+    if (tau_init) {
+        const uint64_t fb = __ballot(valid && !(t[K - 1] < t0));
+        if (lane == 0) {
+            seed_redo[wave] = fb ? 1u : 0u;
+            if (fb) atomicAdd(&seed_fallbacks[wave & (kStatSlots - 1)], static_cast<unsigned long long>(__popcll(fb)));
+        }
+    }
 
     if (stats) {
         unsigned long long v = n_acc;
@@ -795,10 +812,17 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     }
     if (leaf_size < 1 || leaf_size > kWave) return hipErrorInvalidValue;
     const unsigned bound_blocks = static_cast<unsigned>((waves + kBoundWaves - 1) / kBoundWaves);
-    knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size,
-                                                                   init_leaves, scratch.tau, scratch.wleaf,
-                                                                   scratch.wleaf_n, stats);
+    knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
+        spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n, stats,
+        scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks);
     ASM_HIP_CHECK(hipGetLastError());
+    if (scratch.tau_init) {
+        // Wavefronts with a lane whose learned bound was too small: again, unbounded.
+        knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
+            spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
+            stats, nullptr, scratch.seed_redo, nullptr, nullptr);
+        ASM_HIP_CHECK(hipGetLastError());
+    }
     const unsigned col_blocks = static_cast<unsigned>((waves + kCollectWaves - 1) / kCollectWaves);
 #define ASM_COLLECT(M, PAD, ALL)                                                                                \
     knn_collect_kernel<kSlots, M, PAD, ALL><<<col_blocks, kCollectThreads, 0, s>>>(                             \

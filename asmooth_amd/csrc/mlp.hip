@@ -1,0 +1,574 @@
+// Learned seed radius: a 27-64-64-1 multilayer perceptron (GELU) that maps
+// ln(CIC density) over the 3x3x3 cells around a particle, on a 64^3 mesh of
+// the tree cube, to ln(h / s), s the mean particle spacing of the set.  It
+// has no counterpart in the reference; the exact search stays the judge (a
+// predicted radius only bounds the search, and a lane whose radius holds
+// fewer than 33 particles is searched again without it).
+//
+// Training runs on the device with these kernels: a per-sample forward pass
+// that keeps the pre-activations, a per-sample backward pass to the
+// pre-activation gradients, weight gradients as sums over the batch (one
+// thread per parameter, fixed summation order, no atomics), and Adam.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/asmooth.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace asm_hip {
+namespace {
+
+constexpr int kIn = kMlpIn;
+constexpr int kH = kMlpHidden;
+// Parameter layout: W1[kIn][kH], b1[kH], W2[kH][kH], b2[kH], w3[kH], b3.
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kIn * kH;
+constexpr int kOffW2 = kOffB1 + kH;
+constexpr int kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH;
+constexpr int kOffB3 = kOffW3 + kH;
+static_assert(kOffB3 + 1 == kMlpParams, "parameter layout");
+
+constexpr float kGeluC = 0.7978845608028654f;   // sqrt(2 / pi)
+constexpr float kGeluA = 0.044715f;
+
+// GELU in its tanh form, 0.5 z (1 + tanh(c (z + a z^3))), with
+// tanh(u) = 1 - 2 / (exp(2u) + 1).
+__device__ __forceinline__ float gelu_tanh_arg(float z, float& th)
+{
+    const float u = kGeluC * (z + kGeluA * z * z * z);
+    th = 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
+    return 0.5f * z * (1.0f + th);
+}
+__device__ __forceinline__ float gelu(float z)
+{
+    float th;
+    return gelu_tanh_arg(z, th);
+}
+// d gelu / dz
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    float th;
+    (void)gelu_tanh_arg(z, th);
+    return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
+}
+
+// Forward pass of one sample held by the calling lane; parameters in LDS.
+// Optionally keeps the pre-activations z1, z2 (kH each) at the given rows.
+__device__ __forceinline__ float mlp_eval(const float* __restrict__ P, const float (&x)[kIn], float* z1_out,
+                                          float* z2_out)
+{
+    float h1[kH];
+#pragma unroll
+    for (int i = 0; i < kH; ++i) {
+        float z = P[kOffB1 + i];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) z += x[k] * P[kOffW1 + k * kH + i];
+        if (z1_out) z1_out[i] = z;
+        h1[i] = gelu(z);
+    }
+    float y = P[kOffB3];
+    for (int j = 0; j < kH; ++j) {
+        float z = P[kOffB2 + j];
+#pragma unroll
+        for (int i = 0; i < kH; ++i) z += h1[i] * P[kOffW2 + i * kH + j];
+        if (z2_out) z2_out[j] = z;
+        y += P[kOffW3 + j] * gelu(z);
+    }
+    return y;
+}
+
+__device__ __forceinline__ void load_params(float* sp, const float* __restrict__ params)
+{
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) sp[i] = params[i];
+    __syncthreads();
+}
+
+// y[b] = f(x[idx[b]]) (idx may be null); keeps z1, z2 when non-null.
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const float* __restrict__ params, const float* __restrict__ x,
+                                                          const int64_t* __restrict__ idx, int64_t nb,
+                                                          float* __restrict__ y, float* __restrict__ z1,
+                                                          float* __restrict__ z2)
+{
+    __shared__ float sp[kMlpParams];
+    load_params(sp, params);
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int64_t row = idx ? idx[b] : b;
+    float xv[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) xv[k] = x[row * kIn + k];
+    y[b] = mlp_eval(sp, xv, z1 ? z1 + b * kH : nullptr, z2 ? z2 + b * kH : nullptr);
+}
+
+// Per sample: dy = 2 (y - t) / nb (mean squared error), dz2 = dy w3 g'(z2),
+// dz1 = (W2 dz2) g'(z1); z1, z2 are overwritten by g(z1), g(z2) (the
+// activations the weight gradients need).  loss_part[b] = (y - t)^2.
+__global__ __launch_bounds__(256) void mlp_backward_kernel(const float* __restrict__ params,
+                                                           const float* __restrict__ t, const int64_t* __restrict__ idx,
+                                                           int64_t nb, const float* __restrict__ y,
+                                                           float* __restrict__ z1, float* __restrict__ z2,
+                                                           float* __restrict__ dz1, float* __restrict__ dz2,
+                                                           float* __restrict__ dy, float* __restrict__ loss_part)
+{
+    __shared__ float sp[kMlpParams];
+    load_params(sp, params);
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int64_t row = idx ? idx[b] : b;
+    const float r = y[b] - t[row];
+    loss_part[b] = r * r;
+    const float g = 2.0f * r / static_cast<float>(nb);
+    dy[b] = g;
+    float d2[kH];
+#pragma unroll
+    for (int j = 0; j < kH; ++j) {
+        const float z = z2[b * kH + j];
+        d2[j] = g * sp[kOffW3 + j] * gelu_grad(z);
+        dz2[b * kH + j] = d2[j];
+        z2[b * kH + j] = gelu(z);
+    }
+    for (int i = 0; i < kH; ++i) {
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kH; ++j) s += sp[kOffW2 + i * kH + j] * d2[j];
+        const float z = z1[b * kH + i];
+        dz1[b * kH + i] = s * gelu_grad(z);
+        z1[b * kH + i] = gelu(z);
+    }
+}
+
+// One thread per parameter: its gradient as a sum over the batch, in order.
+// h1, h2 are the activations (see mlp_backward_kernel).
+__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float* __restrict__ x, const int64_t* __restrict__ idx,
+                                                        int64_t nb, const float* __restrict__ h1,
+                                                        const float* __restrict__ h2, const float* __restrict__ dz1,
+                                                        const float* __restrict__ dz2, const float* __restrict__ dy,
+                                                        float* __restrict__ grad)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    double s = 0.0;
+    if (p < kOffB1) {
+        const int k = p / kH, i = p % kH;
+        for (int64_t b = 0; b < nb; ++b) s += static_cast<double>(x[(idx ? idx[b] : b) * kIn + k]) * dz1[b * kH + i];
+    } else if (p < kOffW2) {
+        const int i = p - kOffB1;
+        for (int64_t b = 0; b < nb; ++b) s += dz1[b * kH + i];
+    } else if (p < kOffB2) {
+        const int i = (p - kOffW2) / kH, j = (p - kOffW2) % kH;
+        for (int64_t b = 0; b < nb; ++b) s += static_cast<double>(h1[b * kH + i]) * dz2[b * kH + j];
+    } else if (p < kOffW3) {
+        const int j = p - kOffB2;
+        for (int64_t b = 0; b < nb; ++b) s += dz2[b * kH + j];
+    } else if (p < kOffB3) {
+        const int j = p - kOffW3;
+        for (int64_t b = 0; b < nb; ++b) s += static_cast<double>(h2[b * kH + j]) * dy[b];
+    } else {
+        for (int64_t b = 0; b < nb; ++b) s += dy[b];
+    }
+    grad[p] = static_cast<float>(s);
+}
+
+__global__ void adam_kernel(float* __restrict__ params, const float* __restrict__ grad, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float beta1, float beta2, float eps, float bc1,
+                            float bc2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const float g = grad[p];
+    const float mp = beta1 * m[p] + (1.0f - beta1) * g;
+    const float vp = beta2 * v[p] + (1.0f - beta2) * g * g;
+    m[p] = mp;
+    v[p] = vp;
+    params[p] -= lr * (mp / bc1) / (sqrtf(vp / bc2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Features on the smoothing context's sorted particles.
+
+// Cloud-in-cell counts on an nf^3 mesh of the cube [left, left + nf/inv).
+__global__ void cic_kernel(const float4* __restrict__ spos, int64_t n, float left, float inv_cell, int nf,
+                           float* __restrict__ grid)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = spos[i];
+    const float u[3] = {(p.x - left) * inv_cell - 0.5f, (p.y - left) * inv_cell - 0.5f, (p.z - left) * inv_cell - 0.5f};
+    int c0[3];
+    float w1[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float f = floorf(u[d]);
+        c0[d] = static_cast<int>(f);
+        w1[d] = u[d] - f;
+    }
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int cx = min(max(c0[0] + dx, 0), nf - 1);
+                const int cy = min(max(c0[1] + dy, 0), nf - 1);
+                const int cz = min(max(c0[2] + dz, 0), nf - 1);
+                const float w = (dx ? w1[0] : 1.0f - w1[0]) * (dy ? w1[1] : 1.0f - w1[1]) *
+                                (dz ? w1[2] : 1.0f - w1[2]);
+                atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], w);
+            }
+}
+
+// grid <- ln(grid / mean + kLnFloor).
+__global__ void ln_grid_kernel(float* __restrict__ grid, int64_t ncell, float inv_mean)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < ncell) grid[i] = logf(grid[i] * inv_mean + kMlpLnFloor);
+}
+
+__device__ __forceinline__ void gather_features(const float* __restrict__ lngrid, int nf, float left, float inv_cell,
+                                                float px, float py, float pz, float (&x)[kIn])
+{
+    const int cx = min(max(static_cast<int>(floorf((px - left) * inv_cell)), 0), nf - 1);
+    const int cy = min(max(static_cast<int>(floorf((py - left) * inv_cell)), 0), nf - 1);
+    const int cz = min(max(static_cast<int>(floorf((pz - left) * inv_cell)), 0), nf - 1);
+    int k = 0;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int gx = min(max(cx + dx, 0), nf - 1);
+                const int gy = min(max(cy + dy, 0), nf - 1);
+                const int gz = min(max(cz + dz, 0), nf - 1);
+                x[k++] = lngrid[(static_cast<int64_t>(gz) * nf + gy) * nf + gx];
+            }
+}
+
+// Features of the particles at sorted positions sidx[0..m).
+__global__ void features_kernel(const float4* __restrict__ spos, const int32_t* __restrict__ sidx, int64_t m,
+                                const float* __restrict__ lngrid, int nf, float left, float inv_cell,
+                                float* __restrict__ x)
+{
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= m) return;
+    const float4 p = spos[sidx[b]];
+    float xv[kIn];
+    gather_features(lngrid, nf, left, inv_cell, p.x, p.y, p.z, xv);
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) x[b * kIn + k] = xv[k];
+}
+
+// tau_init[q] = (exp(f(x_q)) * spacing * (1 + margin))^2 for every sorted q.
+__global__ __launch_bounds__(256) void seed_bound_kernel(const float4* __restrict__ spos, int64_t n,
+                                                         const float* __restrict__ lngrid, int nf, float left,
+                                                         float inv_cell, const float* __restrict__ params,
+                                                         float scale, float* __restrict__ tau_init)
+{
+    __shared__ float sp[kMlpParams];
+    load_params(sp, params);
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float4 p = spos[q];
+    float xv[kIn];
+    gather_features(lngrid, nf, left, inv_cell, p.x, p.y, p.z, xv);
+    const float r = __expf(mlp_eval(sp, xv, nullptr, nullptr)) * scale;
+    tau_init[q] = r * r;
+}
+
+// Training samples: sorted positions of caller indices, and ln(h / spacing).
+__global__ void seed_index_kernel(const int32_t* __restrict__ inv_perm, const int64_t* __restrict__ idx, int64_t m,
+                                  int32_t* __restrict__ sidx)
+{
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b < m) sidx[b] = inv_perm[idx[b]];
+}
+
+__global__ void seed_target_kernel(const float* __restrict__ rk_sorted, const int32_t* __restrict__ sidx, int64_t m,
+                                   float inv_spacing, float* __restrict__ target)
+{
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b < m) target[b] = logf(rk_sorted[sidx[b]] * inv_spacing);
+}
+
+}  // namespace
+
+hipError_t launch_seed_samples(const float4* spos, const float* rk_sorted, const int32_t* inv_perm,
+                               const int64_t* idx, int64_t m, const float* lngrid, float left, float right,
+                               float inv_spacing, int32_t* sidx, float* x, float* target, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>((m + 255) / 256);
+    seed_index_kernel<<<blocks, 256, 0, s>>>(inv_perm, idx, m, sidx);
+    ASM_HIP_CHECK(hipGetLastError());
+    ASM_HIP_CHECK(launch_seed_features(spos, sidx, m, lngrid, left, right, x, s));
+    seed_target_kernel<<<blocks, 256, 0, s>>>(rk_sorted, sidx, m, inv_spacing, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_grid(const float4* spos, int64_t n, float left, float right, float* grid, hipStream_t s)
+{
+    const int nf = kMlpGrid;
+    const int64_t ncell = static_cast<int64_t>(nf) * nf * nf;
+    ASM_HIP_CHECK(launch_fill_zero_f32(grid, ncell, s));
+    const float inv_cell = static_cast<float>(nf) / (right - left);
+    if (n > 0) cic_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(spos, n, left, inv_cell, nf, grid);
+    ASM_HIP_CHECK(hipGetLastError());
+    ln_grid_kernel<<<static_cast<unsigned>((ncell + 255) / 256), 256, 0, s>>>(grid, ncell,
+                                                                               static_cast<float>(ncell) /
+                                                                                   static_cast<float>(n > 0 ? n : 1));
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_features(const float4* spos, const int32_t* sidx, int64_t m, const float* lngrid, float left,
+                                float right, float* x, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    const int nf = kMlpGrid;
+    features_kernel<<<static_cast<unsigned>((m + 255) / 256), 256, 0, s>>>(spos, sidx, m, lngrid, nf, left,
+                                                                           static_cast<float>(nf) / (right - left), x);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_bound(const float4* spos, int64_t n, const float* lngrid, float left, float right,
+                             const float* params, float scale, float* tau_init, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int nf = kMlpGrid;
+    seed_bound_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(
+        spos, n, lngrid, nf, left, static_cast<float>(nf) / (right - left), params, scale, tau_init);
+    return hipGetLastError();
+}
+
+}  // namespace asm_hip
+
+using namespace asm_hip;
+
+struct asm_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    float* params = nullptr;   // device, kMlpParams
+    float* grad = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    int64_t adam_t = 0;
+};
+
+namespace {
+
+int mlp_fail(asm_mlp* m, int code, const std::string& msg)
+{
+    if (m) m->err = msg;
+    return code;
+}
+
+#define MLPC(expr)                                                                     \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) return mlp_fail(m, ASM_ERR_HIP, hipGetErrorString(_e));  \
+    } while (0)
+
+// Device scratch of one batch.
+struct Batch {
+    float *x = nullptr, *t = nullptr, *y = nullptr, *z1 = nullptr, *z2 = nullptr, *dz1 = nullptr, *dz2 = nullptr,
+          *dy = nullptr, *loss = nullptr;
+    int64_t* idx = nullptr;
+    ~Batch()
+    {
+        for (void* p : {static_cast<void*>(x), static_cast<void*>(t), static_cast<void*>(y), static_cast<void*>(z1),
+                        static_cast<void*>(z2), static_cast<void*>(dz1), static_cast<void*>(dz2),
+                        static_cast<void*>(dy), static_cast<void*>(loss), static_cast<void*>(idx)})
+            if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(int64_t nrows, int64_t nb)
+    {
+        hipError_t e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&x), sizeof(float) * nrows * kIn)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&t), sizeof(float) * nrows)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&y), sizeof(float) * nb)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&z1), sizeof(float) * nb * kH)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&z2), sizeof(float) * nb * kH)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&dz1), sizeof(float) * nb * kH)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&dz2), sizeof(float) * nb * kH)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&dy), sizeof(float) * nb)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&loss), sizeof(float) * nb)) != hipSuccess) return e;
+        return hipMalloc(reinterpret_cast<void**>(&idx), sizeof(int64_t) * nb);
+    }
+};
+
+// Gradient of the mean squared error over rows idx[0..nb) (or 0..nb) into
+// m->grad; per-sample squared errors into B.loss.
+hipError_t batch_grad(asm_mlp* m, Batch& B, const int64_t* idx, int64_t nb)
+{
+    const unsigned blocks = static_cast<unsigned>((nb + 255) / 256);
+    mlp_forward_kernel<<<blocks, 256, 0, m->stream>>>(m->params, B.x, idx, nb, B.y, B.z1, B.z2);
+    ASM_HIP_CHECK(hipGetLastError());
+    mlp_backward_kernel<<<blocks, 256, 0, m->stream>>>(m->params, B.t, idx, nb, B.y, B.z1, B.z2, B.dz1, B.dz2, B.dy,
+                                                       B.loss);
+    ASM_HIP_CHECK(hipGetLastError());
+    mlp_wgrad_kernel<<<(kMlpParams + 255) / 256, 256, 0, m->stream>>>(B.x, idx, nb, B.z1, B.z2, B.dz1, B.dz2, B.dy,
+                                                                       m->grad);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int asm_mlp_param_count(void)
+{
+    return kMlpParams;
+}
+
+int asm_mlp_create(asm_mlp** out, int device)
+{
+    if (!out) return ASM_ERR_ARG;
+    *out = nullptr;
+    asm_mlp* m = new asm_mlp();
+    m->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreate(&m->stream);
+    for (float** p : {&m->params, &m->grad, &m->adam_m, &m->adam_v})
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), sizeof(float) * kMlpParams);
+    for (float* p : {m->params, m->grad, m->adam_m, m->adam_v})
+        if (e == hipSuccess) e = launch_fill_zero_f32(p, kMlpParams, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) {
+        asm_mlp_destroy(m);
+        return ASM_ERR_HIP;
+    }
+    *out = m;
+    return ASM_OK;
+}
+
+void asm_mlp_destroy(asm_mlp* m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    for (float* p : {m->params, m->grad, m->adam_m, m->adam_v})
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+const char* asm_mlp_last_error(const asm_mlp* m)
+{
+    return m ? m->err.c_str() : "null model";
+}
+
+int asm_mlp_set_params(asm_mlp* m, const float* params)
+{
+    if (!m || !params) return ASM_ERR_ARG;
+    MLPC(hipSetDevice(m->device));
+    MLPC(hipMemcpyAsync(m->params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
+    MLPC(launch_fill_zero_f32(m->adam_m, kMlpParams, m->stream));
+    MLPC(launch_fill_zero_f32(m->adam_v, kMlpParams, m->stream));
+    m->adam_t = 0;
+    MLPC(hipStreamSynchronize(m->stream));
+    return ASM_OK;
+}
+
+int asm_mlp_get_params(asm_mlp* m, float* params)
+{
+    if (!m || !params) return ASM_ERR_ARG;
+    MLPC(hipSetDevice(m->device));
+    MLPC(hipMemcpyAsync(params, m->params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
+    MLPC(hipStreamSynchronize(m->stream));
+    return ASM_OK;
+}
+
+int asm_mlp_forward(asm_mlp* m, const float* x, int64_t n, float* y)
+{
+    if (!m || (n > 0 && (!x || !y)) || n < 0) return ASM_ERR_ARG;
+    if (n == 0) return ASM_OK;
+    MLPC(hipSetDevice(m->device));
+    float *dx = nullptr, *dy = nullptr;
+    MLPC(hipMalloc(reinterpret_cast<void**>(&dx), sizeof(float) * n * kIn));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dy), sizeof(float) * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+        mlp_forward_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, m->stream>>>(m->params, dx, nullptr, n,
+                                                                                         dy, nullptr, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) return mlp_fail(m, ASM_ERR_HIP, hipGetErrorString(e));
+    return ASM_OK;
+}
+
+int asm_mlp_loss_grad(asm_mlp* m, const float* x, const float* t, int64_t n, double* loss, float* grad)
+{
+    if (!m || !x || !t || n <= 0) return ASM_ERR_ARG;
+    MLPC(hipSetDevice(m->device));
+    Batch B;
+    MLPC(B.alloc(n, n));
+    MLPC(hipMemcpyAsync(B.x, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice, m->stream));
+    MLPC(hipMemcpyAsync(B.t, t, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    MLPC(batch_grad(m, B, nullptr, n));
+    std::vector<float> lp(static_cast<size_t>(n));
+    MLPC(hipMemcpyAsync(lp.data(), B.loss, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+    if (grad) MLPC(hipMemcpyAsync(grad, m->grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
+    MLPC(hipStreamSynchronize(m->stream));
+    if (loss) {
+        double s = 0.0;
+        for (float v : lp) s += v;
+        *loss = s / static_cast<double>(n);
+    }
+    return ASM_OK;
+}
+
+int asm_mlp_train(asm_mlp* m, const float* x, const float* t, int64_t n, int epochs, int batch, float lr,
+                  uint64_t seed, float* epoch_loss)
+{
+    if (!m || !x || !t || n <= 0 || epochs <= 0 || batch <= 0) return ASM_ERR_ARG;
+    MLPC(hipSetDevice(m->device));
+    Batch B;
+    const int64_t nb = batch < n ? batch : n;
+    MLPC(B.alloc(n, nb));
+    MLPC(hipMemcpyAsync(B.x, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice, m->stream));
+    MLPC(hipMemcpyAsync(B.t, t, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    std::vector<int64_t> perm(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) perm[static_cast<size_t>(i)] = i;
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto rnd = [&st]() {   // xorshift64*
+        st ^= st >> 12;
+        st ^= st << 25;
+        st ^= st >> 27;
+        return st * 0x2545F4914F6CDD1Dull;
+    };
+    std::vector<float> lp(static_cast<size_t>(nb));
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int64_t i = n - 1; i > 0; --i) std::swap(perm[static_cast<size_t>(i)], perm[rnd() % (i + 1)]);
+        double esum = 0.0;
+        int64_t ecount = 0;
+        for (int64_t b0 = 0; b0 + nb <= n; b0 += nb) {
+            MLPC(hipMemcpyAsync(B.idx, perm.data() + b0, sizeof(int64_t) * nb, hipMemcpyHostToDevice, m->stream));
+            MLPC(batch_grad(m, B, B.idx, nb));
+            ++m->adam_t;
+            const float bc1 = 1.0f - powf(beta1, static_cast<float>(m->adam_t));
+            const float bc2 = 1.0f - powf(beta2, static_cast<float>(m->adam_t));
+            adam_kernel<<<(kMlpParams + 255) / 256, 256, 0, m->stream>>>(m->params, m->grad, m->adam_m, m->adam_v, lr,
+                                                                         beta1, beta2, eps, bc1, bc2);
+            MLPC(hipGetLastError());
+            MLPC(hipMemcpyAsync(lp.data(), B.loss, sizeof(float) * nb, hipMemcpyDeviceToHost, m->stream));
+            MLPC(hipStreamSynchronize(m->stream));
+            for (float v : lp) esum += v;
+            ecount += nb;
+        }
+        if (epoch_loss) epoch_loss[ep] = static_cast<float>(esum / static_cast<double>(ecount > 0 ? ecount : 1));
+    }
+    return ASM_OK;
+}
+
+}  // extern "C"

@@ -198,6 +198,40 @@ int asm_write_mesh_separate(const char* base, int inode, const float* mesh, int 
 int asm_autotune(asm_context* ctx, const int* leaf_sizes, int n_leaf, const int* ncs, int n_nc, int sign,
                  int* leaf_out, int* tiles_out);
 
+/* Learned seed radius (no counterpart in the reference).  A 27-64-64-1
+ * multilayer perceptron with GELU activations maps ln(CIC density / mean +
+ * 1e-3) over the 3x3x3 cells around a particle, on a 64^3 mesh of the tree
+ * cube, to ln(h / s), s the mean particle spacing (tree cube volume over
+ * particle count, cube root).  Parameters, asm_mlp_param_count() floats:
+ * W1[27][64], b1[64], W2[64][64], b2[64], w3[64], b3.  The model is trained
+ * on the device (mean squared error, Adam). */
+typedef struct asm_mlp asm_mlp;
+int asm_mlp_param_count(void);
+int asm_mlp_create(asm_mlp** m, int device);
+void asm_mlp_destroy(asm_mlp* m);
+const char* asm_mlp_last_error(const asm_mlp* m);
+int asm_mlp_set_params(asm_mlp* m, const float* params);   /* also resets the optimiser state */
+int asm_mlp_get_params(asm_mlp* m, float* params);
+/* y[i] = f(x[i]), x: n x 27 floats (host). */
+int asm_mlp_forward(asm_mlp* m, const float* x, int64_t n, float* y);
+/* Mean squared error of f(x) against t and its gradient (host arrays). */
+int asm_mlp_loss_grad(asm_mlp* m, const float* x, const float* t, int64_t n, double* loss, float* grad);
+/* Adam (beta 0.9, 0.999) over shuffled batches; epoch_loss[e] = mean loss
+ * of epoch e before each step. */
+int asm_mlp_train(asm_mlp* m, const float* x, const float* t, int64_t n, int epochs, int batch, float lr,
+                  uint64_t seed, float* epoch_loss);
+/* Features (m x 27) and targets ln(h / s) of particles idx[0..m) (caller
+ * order), after a search: training samples. */
+int asm_seed_samples(asm_context* ctx, const int64_t* idx, int64_t m, float* x, float* target);
+/* Use the model in later searches: a particle's search starts bounded by the
+ * radius (1 + margin) * s * exp(f(x)), and a particle whose radius holds
+ * fewer than 33 particles is searched again without it.  Results are those
+ * of the plain search, bit for bit.  m = NULL switches it off. */
+int asm_set_seed_model(asm_context* ctx, const asm_mlp* m, float margin);
+/* Of the last search with a seed model: out[0] particles searched again,
+ * out[1] wavefronts (64 sorted particles) searched again, out[2] particles. */
+int asm_get_seed_stats(asm_context* ctx, int64_t out[3]);
+
 /* Timers (HIP events on the context stream) when ASM_OPT_PROFILE is set. */
 int asm_timer_count(void);
 const char* asm_timer_name(int i);
