@@ -102,7 +102,7 @@ struct asm_context {
     // learned seed radius
     bool seed_on = false;
     float seed_margin = 0.0f;
-    DevBuf<float> seed_params, seed_grid, tau_init;
+    DevBuf<float> seed_params, seed_grid, seed_rcell, tau_init;
     DevBuf<uint32_t> seed_redo;
     DevBuf<unsigned long long> seed_fb;
     bool seed_used = false;   // last search used the model
@@ -397,6 +397,7 @@ int asm_neighbor_search(asm_context* ctx, int k)
     ctx->seed_used = false;
     if (bound_collect && ctx->seed_on) {
         HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));
+        HIPC(ctx->seed_rcell.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));
         HIPC(ctx->tau_init.ensure(n));
         HIPC(ctx->seed_redo.ensure(nwaves));
         HIPC(ctx->seed_fb.ensure(kStatSlots));
@@ -406,7 +407,7 @@ int asm_neighbor_search(asm_context* ctx, int k)
         HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->seed_fb.p), 2 * kStatSlots, s));
         HIPC(launch_seed_grid(ctx->spos.p, n, ctx->tree_left, ctx->tree_right, ctx->seed_grid.p, s));
         HIPC(launch_seed_bound(ctx->spos.p, n, ctx->seed_grid.p, ctx->tree_left, ctx->tree_right, ctx->seed_params.p,
-                               spacing * (1.0f + ctx->seed_margin), ctx->tau_init.p, s));
+                               spacing * (1.0f + ctx->seed_margin), ctx->seed_rcell.p, ctx->tau_init.p, s));
         kscr.tau_init = ctx->tau_init.p;
         kscr.seed_redo = ctx->seed_redo.p;
         kscr.seed_fallbacks = ctx->seed_fb.p;

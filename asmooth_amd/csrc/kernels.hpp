@@ -102,9 +102,10 @@ hipError_t launch_seed_features(const float4* spos, const int32_t* sidx, int64_t
 hipError_t launch_seed_samples(const float4* spos, const float* rk_sorted, const int32_t* inv_perm,
                                const int64_t* idx, int64_t m, const float* lngrid, float left, float right,
                                float inv_spacing, int32_t* sidx, float* x, float* target, hipStream_t s);
-// tau_init[q] = (exp(f(x_q)) * scale)^2, scale = spacing * (1 + margin).
+// rcell[c] = exp(f(x_c)) * scale per mesh cell (kMlpGrid^3 floats), scale =
+// spacing * (1 + margin); tau_init[q] = rcell[cell of q]^2.
 hipError_t launch_seed_bound(const float4* spos, int64_t n, const float* lngrid, float left, float right,
-                             const float* params, float scale, float* tau_init, hipStream_t s);
+                             const float* params, float scale, float* rcell, float* tau_init, hipStream_t s);
 
 // ---- deposit.hip -------------------------------------------------------------
 struct DepositArgs {

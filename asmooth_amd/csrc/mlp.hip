@@ -43,7 +43,7 @@ constexpr float kGeluA = 0.044715f;
 __device__ __forceinline__ float gelu_tanh_arg(float z, float& th)
 {
     const float u = kGeluC * (z + kGeluA * z * z * z);
-    th = 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
+    th = 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * u) + 1.0f);
     return 0.5f * z * (1.0f + th);
 }
 __device__ __forceinline__ float gelu(float z)
@@ -194,34 +194,84 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
 // Features on the smoothing context's sorted particles.
 
 // Cloud-in-cell counts on an nf^3 mesh of the cube [left, left + nf/inv).
-__global__ void cic_kernel(const float4* __restrict__ spos, int64_t n, float left, float inv_cell, int nf,
-                           float* __restrict__ grid)
+// Particles arrive in Morton order, so the 1024 of a workgroup cover a
+// small block of cells: the workgroup accumulates into an LDS window of
+// kCicWin^3 cells placed at the block's lowest stencil corner and adds the
+// window's non-zero cells to the mesh; stencils outside the window (a block
+// spanning a Morton jump) go to the mesh directly.
+constexpr int kCicThreads = 256;
+constexpr int kCicPerThread = 4;
+constexpr int kCicWin = 12;
+
+__device__ __forceinline__ void cic_stencil(float4 p, float left, float inv_cell, int (&c0)[3], float (&w1)[3])
 {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = spos[i];
     const float u[3] = {(p.x - left) * inv_cell - 0.5f, (p.y - left) * inv_cell - 0.5f, (p.z - left) * inv_cell - 0.5f};
-    int c0[3];
-    float w1[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const float f = floorf(u[d]);
         c0[d] = static_cast<int>(f);
         w1[d] = u[d] - f;
     }
+}
+
+__global__ __launch_bounds__(kCicThreads) void cic_kernel(const float4* __restrict__ spos, int64_t n, float left,
+                                                          float inv_cell, int nf, float* __restrict__ grid)
+{
+    __shared__ float win[kCicWin * kCicWin * kCicWin];
+    __shared__ int corner[3];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kCicThreads * kCicPerThread;
+    for (int c = threadIdx.x; c < kCicWin * kCicWin * kCicWin; c += kCicThreads) win[c] = 0.0f;
+    if (threadIdx.x < 3) corner[threadIdx.x] = 0x7fffffff;
+    __syncthreads();
+    int c0[kCicPerThread][3];
+    float w1[kCicPerThread][3];
+    bool act[kCicPerThread];
 #pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
+    for (int r = 0; r < kCicPerThread; ++r) {
+        const int64_t i = base + static_cast<int64_t>(r) * kCicThreads + threadIdx.x;
+        act[r] = i < n;
+        if (act[r]) {
+            cic_stencil(spos[i], left, inv_cell, c0[r], w1[r]);
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
+            for (int d = 0; d < 3; ++d) atomicMin(&corner[d], c0[r][d]);
+        }
+    }
+    __syncthreads();
+    const int k0[3] = {corner[0], corner[1], corner[2]};
 #pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                const int cx = min(max(c0[0] + dx, 0), nf - 1);
-                const int cy = min(max(c0[1] + dy, 0), nf - 1);
-                const int cz = min(max(c0[2] + dz, 0), nf - 1);
-                const float w = (dx ? w1[0] : 1.0f - w1[0]) * (dy ? w1[1] : 1.0f - w1[1]) *
-                                (dz ? w1[2] : 1.0f - w1[2]);
-                atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], w);
-            }
+    for (int r = 0; r < kCicPerThread; ++r) {
+        if (!act[r]) continue;
+        const bool inwin = c0[r][0] + 1 - k0[0] < kCicWin && c0[r][1] + 1 - k0[1] < kCicWin &&
+                           c0[r][2] + 1 - k0[2] < kCicWin;
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const float w = (dx ? w1[r][0] : 1.0f - w1[r][0]) * (dy ? w1[r][1] : 1.0f - w1[r][1]) *
+                                    (dz ? w1[r][2] : 1.0f - w1[r][2]);
+                    if (inwin) {
+                        atomicAdd(&win[((c0[r][2] + dz - k0[2]) * kCicWin + (c0[r][1] + dy - k0[1])) * kCicWin +
+                                       (c0[r][0] + dx - k0[0])],
+                                  w);
+                    } else {
+                        const int cx = min(max(c0[r][0] + dx, 0), nf - 1);
+                        const int cy = min(max(c0[r][1] + dy, 0), nf - 1);
+                        const int cz = min(max(c0[r][2] + dz, 0), nf - 1);
+                        atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], w);
+                    }
+                }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < kCicWin * kCicWin * kCicWin; c += kCicThreads) {
+        const float v = win[c];
+        if (v == 0.0f) continue;
+        const int cx = min(max(k0[0] + c % kCicWin, 0), nf - 1);
+        const int cy = min(max(k0[1] + (c / kCicWin) % kCicWin, 0), nf - 1);
+        const int cz = min(max(k0[2] + c / (kCicWin * kCicWin), 0), nf - 1);
+        atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], v);
+    }
 }
 
 // grid <- ln(grid / mean + kLnFloor).
@@ -265,20 +315,45 @@ __global__ void features_kernel(const float4* __restrict__ spos, const int32_t* 
     for (int k = 0; k < kIn; ++k) x[b * kIn + k] = xv[k];
 }
 
-// tau_init[q] = (exp(f(x_q)) * spacing * (1 + margin))^2 for every sorted q.
-__global__ __launch_bounds__(256) void seed_bound_kernel(const float4* __restrict__ spos, int64_t n,
-                                                         const float* __restrict__ lngrid, int nf, float left,
-                                                         float inv_cell, const float* __restrict__ params,
-                                                         float scale, float* __restrict__ tau_init)
+// The features of a particle are those of its cell, so the model is
+// evaluated once per cell: rcell[c] = exp(f(x_c)) * scale.
+__global__ __launch_bounds__(256) void seed_cell_kernel(const float* __restrict__ lngrid, int nf,
+                                                        const float* __restrict__ params, float scale,
+                                                        float* __restrict__ rcell)
 {
     __shared__ float sp[kMlpParams];
     load_params(sp, params);
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t ncell = static_cast<int64_t>(nf) * nf * nf;
+    if (c >= ncell) return;
+    const int cx = static_cast<int>(c % nf), cy = static_cast<int>((c / nf) % nf), cz = static_cast<int>(c / nf / nf);
+    float xv[kIn];
+    int k = 0;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int gx = min(max(cx + dx, 0), nf - 1);
+                const int gy = min(max(cy + dy, 0), nf - 1);
+                const int gz = min(max(cz + dz, 0), nf - 1);
+                xv[k++] = lngrid[(static_cast<int64_t>(gz) * nf + gy) * nf + gx];
+            }
+    rcell[c] = __expf(mlp_eval(sp, xv, nullptr, nullptr)) * scale;
+}
+
+// tau_init[q] = rcell[cell of q]^2.
+__global__ void seed_bound_kernel(const float4* __restrict__ spos, int64_t n, const float* __restrict__ rcell, int nf,
+                                  float left, float inv_cell, float* __restrict__ tau_init)
+{
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const float4 p = spos[q];
-    float xv[kIn];
-    gather_features(lngrid, nf, left, inv_cell, p.x, p.y, p.z, xv);
-    const float r = __expf(mlp_eval(sp, xv, nullptr, nullptr)) * scale;
+    const int cx = min(max(static_cast<int>(floorf((p.x - left) * inv_cell)), 0), nf - 1);
+    const int cy = min(max(static_cast<int>(floorf((p.y - left) * inv_cell)), 0), nf - 1);
+    const int cz = min(max(static_cast<int>(floorf((p.z - left) * inv_cell)), 0), nf - 1);
+    const float r = rcell[(static_cast<int64_t>(cz) * nf + cy) * nf + cx];
     tau_init[q] = r * r;
 }
 
@@ -318,7 +393,10 @@ hipError_t launch_seed_grid(const float4* spos, int64_t n, float left, float rig
     const int64_t ncell = static_cast<int64_t>(nf) * nf * nf;
     ASM_HIP_CHECK(launch_fill_zero_f32(grid, ncell, s));
     const float inv_cell = static_cast<float>(nf) / (right - left);
-    if (n > 0) cic_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(spos, n, left, inv_cell, nf, grid);
+    const int64_t per_block = static_cast<int64_t>(kCicThreads) * kCicPerThread;
+    if (n > 0)
+        cic_kernel<<<static_cast<unsigned>((n + per_block - 1) / per_block), kCicThreads, 0, s>>>(spos, n, left,
+                                                                                                 inv_cell, nf, grid);
     ASM_HIP_CHECK(hipGetLastError());
     ln_grid_kernel<<<static_cast<unsigned>((ncell + 255) / 256), 256, 0, s>>>(grid, ncell,
                                                                                static_cast<float>(ncell) /
@@ -337,12 +415,15 @@ hipError_t launch_seed_features(const float4* spos, const int32_t* sidx, int64_t
 }
 
 hipError_t launch_seed_bound(const float4* spos, int64_t n, const float* lngrid, float left, float right,
-                             const float* params, float scale, float* tau_init, hipStream_t s)
+                             const float* params, float scale, float* rcell, float* tau_init, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int nf = kMlpGrid;
+    const int64_t ncell = static_cast<int64_t>(nf) * nf * nf;
+    seed_cell_kernel<<<static_cast<unsigned>((ncell + 255) / 256), 256, 0, s>>>(lngrid, nf, params, scale, rcell);
+    ASM_HIP_CHECK(hipGetLastError());
     seed_bound_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(
-        spos, n, lngrid, nf, left, static_cast<float>(nf) / (right - left), params, scale, tau_init);
+        spos, n, rcell, nf, left, static_cast<float>(nf) / (right - left), tau_init);
     return hipGetLastError();
 }
 
