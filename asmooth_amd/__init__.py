@@ -27,7 +27,7 @@ import numpy as np
 __all__ = [
     "PARTICLE_DTYPE", "HALO_DTYPE", "K_SLOTS", "AsmoothError", "Context", "lib",
     "build_buffered_set", "build_node_share", "build_buffered_halos", "read_halo_file", "halo_delta_vir", "read_xv", "write_xv", "write_mesh_separate",
-    "particles_from_xv", "timer_names", "Mlp", "mlp_init_params",
+    "particles_from_xv", "timer_names", "Mlp", "mlp_init_params", "Batch",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -122,6 +122,19 @@ def lib() -> C.CDLL:
         "asm_seed_samples": (i32, [vp, vp, i64, vp, vp]),
         "asm_set_seed_model": (i32, [vp, vp, C.c_float]),
         "asm_get_seed_stats": (i32, [vp, P(i64)]),
+        "asm_batch_create": (i32, [P(vp), i32]),
+        "asm_batch_destroy": (None, [vp]),
+        "asm_batch_last_error": (C.c_char_p, [vp]),
+        "asm_batch_add": (i32, [vp, vp, i64, i64, C.c_float, C.c_float, C.c_float, P(i32), i32, i32]),
+        "asm_batch_size": (i32, [vp]),
+        "asm_batch_set_option": (i32, [vp, i32, i32]),
+        "asm_batch_autotune": (i32, [vp, P(i32), i32, P(i32), P(i32)]),
+        "asm_batch_capture": (i32, [vp, i32]),
+        "asm_batch_launch": (i32, [vp]),
+        "asm_batch_run_sequential": (i32, [vp]),
+        "asm_batch_synchronize": (i32, [vp]),
+        "asm_batch_get_mesh": (i32, [vp, i32, i32, vp, P(f64)]),
+        "asm_batch_get_rk_dens": (i32, [vp, i32, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -396,6 +409,82 @@ class Mlp:
         self._check(self._L.asm_mlp_train(self._h, _ptr(x), _ptr(t), len(x), int(epochs), int(batch), float(lr),
                                           int(seed), _ptr(losses)), "asm_mlp_train")
         return losses
+
+
+class Batch:
+    """Independent node shares run as one recorded HIP graph (asm_batch)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.asm_batch_create(C.byref(h), int(device))
+        if rc != 0:
+            raise AsmoothError(f"asm_batch_create(device={device}) failed with status {rc}")
+        self._h = h
+        self.shares = []
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = self._L.asm_batch_last_error(self._h).decode(errors="replace")
+            raise AsmoothError(f"{what}: status {rc}: {msg}")
+        return rc
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.asm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, particles: np.ndarray, np_local: int, boxsize: float, left: float, right: float,
+            ncs: Sequence[int], sign: int = -1) -> int:
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        nn = (C.c_int32 * len(ncs))(*ncs)
+        i = self._check(self._L.asm_batch_add(self._h, _ptr(p), len(p), int(np_local), float(boxsize), float(left),
+                                              float(right), nn, len(ncs), int(sign)), "asm_batch_add")
+        self.shares.append({"n": len(p), "np_local": int(np_local), "ncs": list(ncs)})
+        return i
+
+    def set_option(self, option: int, value: int) -> None:
+        self._check(self._L.asm_batch_set_option(self._h, option, int(value)), "asm_batch_set_option")
+
+    def autotune(self, leaf_sizes=(16, 24, 32)) -> tuple[int, list[int]]:
+        ls = (C.c_int32 * len(leaf_sizes))(*leaf_sizes)
+        big = max(range(len(self.shares)), key=lambda i: self.shares[i]["n"])
+        tiles = (C.c_int32 * len(self.shares[big]["ncs"]))()
+        leaf = C.c_int32()
+        self._check(self._L.asm_batch_autotune(self._h, ls, len(leaf_sizes), C.byref(leaf), tiles),
+                    "asm_batch_autotune")
+        return leaf.value, list(tiles)
+
+    def capture(self, n_streams: int = 8) -> None:
+        self._check(self._L.asm_batch_capture(self._h, int(n_streams)), "asm_batch_capture")
+
+    def launch(self) -> None:
+        self._check(self._L.asm_batch_launch(self._h), "asm_batch_launch")
+
+    def run_sequential(self) -> None:
+        self._check(self._L.asm_batch_run_sequential(self._h), "asm_batch_run_sequential")
+
+    def synchronize(self) -> None:
+        self._check(self._L.asm_batch_synchronize(self._h), "asm_batch_synchronize")
+
+    def get_mesh(self, i: int, nc: int) -> tuple[np.ndarray, np.ndarray]:
+        mesh = np.empty(6 * nc ** 3, np.float32)
+        tot = (C.c_double * 4)()
+        self._check(self._L.asm_batch_get_mesh(self._h, int(i), int(nc), _ptr(mesh), tot), "asm_batch_get_mesh")
+        return mesh, np.array(list(tot))
+
+    def get_rk_dens(self, i: int) -> tuple[np.ndarray, np.ndarray]:
+        n = self.shares[i]["n"]
+        rk = np.empty(n, np.float32)
+        dens = np.empty(n, np.float32)
+        self._check(self._L.asm_batch_get_rk_dens(self._h, int(i), _ptr(rk), _ptr(dens)), "asm_batch_get_rk_dens")
+        return rk, dens
 
 
 def mlp_init_params(seed: int = 0) -> np.ndarray:

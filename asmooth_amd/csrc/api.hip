@@ -81,6 +81,9 @@ struct asm_context {
     bool have_particles = false;
     bool have_tree = false;
     bool have_search = false;
+    // Set while a batch records its work into a HIP graph: no host reads,
+    // buffers already sized by an eager run of the same work.
+    bool capturing = false;
     DevBuf<ParticleRec> input;   // caller order
     // sort / tree
     DevBuf<uint32_t> keys, vals, keys_tmp, vals_tmp, hist;
@@ -518,9 +521,12 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
         HIPC(hipMemcpyAsync(ctx->tile_start.p, ctx->tile_cnt.p, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToDevice, s));
         HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, s));
     }
-    HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIPC(hipStreamSynchronize(s));
-    HIPC(ctx->list.ensure(total > 0 ? total : 1));
+    if (!ctx->capturing) {
+        HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        HIPC(ctx->list.ensure(total > 0 ? total : 1));
+    }
+    a.list_cap = static_cast<uint32_t>(ctx->list.cap < 0xffffffffull ? ctx->list.cap : 0xffffffffull);
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cur.p, ntiles, s));
@@ -919,6 +925,332 @@ int asm_get_search_stats(asm_context* ctx, uint64_t out[4])
         out[3] += x.inserts;
     }
     return ASM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of independent node shares (one context per share), recorded once
+// into a HIP graph: every share's tree build, search, depositions, FP64 mesh
+// totals and normalisation on its own stream, the streams forked from and
+// joined to one launch stream, so that small shares run side by side.
+
+#define HIPC2(b, expr)                                                                        \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) return batch_fail((b), ASM_ERR_HIP, "%s", hipGetErrorString(_e)); \
+    } while (0)
+
+int batch_fail(struct asm_batch* b, int code, const char* fmt, ...);
+
+struct asm_batch {
+    int device = 0;
+    hipStream_t main = nullptr;
+    std::string err;
+    struct Cube {
+        asm_context* ctx = nullptr;
+        float left = 0.0f, right = 0.0f;
+        int sign = -1;
+        std::vector<int> ncs;
+        int64_t n = 0, np_local = 0;
+    };
+    std::vector<Cube> cubes;
+    DevBuf<double> totals;          // per (cube, nc): sum n, sum halo, sum n^2, sum c (raw)
+    std::vector<size_t> total_off;  // first totals slot of each cube
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int batch_fail(asm_batch* b, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    return code;
+}
+
+namespace {
+
+// One share's work on its context stream; totals at `tot` (4 per nc).
+int run_cube(asm_batch::Cube& c, double* tot)
+{
+    asm_context* ctx = c.ctx;
+    int rc = asm_tree_construct(ctx, c.left, c.right, nullptr);
+    if (!rc) rc = asm_neighbor_search(ctx, kSlots - 1);
+    for (size_t i = 0; i < c.ncs.size() && !rc; ++i) {
+        const int nc = c.ncs[i];
+        rc = deposit_one(ctx, nc, c.sign, 0, nullptr);
+        if (rc) break;
+        MeshDev& m = ctx->meshes[nc];
+        HIPC(ctx->red_partial.ensure(4 * 1024));
+        HIPC(launch_mesh_reduce(m.mesh.p, nc, ctx->red_partial.p, tot + 4 * i, ctx->stream));
+        // coarse_mesh.cpp:151: vol = pow(boxsize/nc, 3) in FP32 (as asm_normalize_mesh)
+        const float vol = std::pow(ctx->boxsize / static_cast<float>(nc), 3.0f);
+        HIPC(launch_normalize(m.mesh.p, nc, vol, ctx->stream));
+    }
+    return rc;
+}
+
+int batch_release_graph(asm_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    return ASM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int asm_batch_create(asm_batch** out, int device)
+{
+    if (!out) return ASM_ERR_ARG;
+    *out = nullptr;
+    asm_batch* b = new asm_batch();
+    b->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return ASM_ERR_HIP;
+    }
+    *out = b;
+    return ASM_OK;
+}
+
+void asm_batch_destroy(asm_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    batch_release_graph(b);
+    for (auto& c : b->cubes) asm_destroy(c.ctx);
+    b->totals.release();
+    if (b->main) (void)hipStreamDestroy(b->main);
+    delete b;
+}
+
+const char* asm_batch_last_error(const asm_batch* b)
+{
+    return b ? b->err.c_str() : "null batch";
+}
+
+int asm_batch_add(asm_batch* b, const asm_particle* particles, int64_t n, int64_t np_local, float boxsize,
+                  float left, float right, const int* ncs, int n_nc, int sign)
+{
+    if (!b || !particles || !ncs || n_nc <= 0) return batch_fail(b, ASM_ERR_ARG, "null argument");
+    batch_release_graph(b);
+    asm_batch::Cube c;
+    int rc = asm_create(&c.ctx, b->device);
+    if (rc) return batch_fail(b, rc, "cannot create a context for share %zu", b->cubes.size());
+    rc = asm_set_particles(c.ctx, particles, n, np_local, boxsize);
+    if (rc) {
+        b->err = asm_last_error(c.ctx);
+        asm_destroy(c.ctx);
+        return rc;
+    }
+    c.left = left;
+    c.right = right;
+    c.sign = sign;
+    c.ncs.assign(ncs, ncs + n_nc);
+    c.n = n;
+    c.np_local = np_local;
+    b->cubes.push_back(c);
+    return static_cast<int>(b->cubes.size() - 1);
+}
+
+int asm_batch_set_option(asm_batch* b, int option, int value)
+{
+    if (!b) return ASM_ERR_ARG;
+    batch_release_graph(b);
+    for (auto& c : b->cubes) {
+        const int rc = asm_set_option(c.ctx, option, value);
+        if (rc) {
+            b->err = asm_last_error(c.ctx);
+            return rc;
+        }
+    }
+    return ASM_OK;
+}
+
+// Autotune on the share with the most particles, applied to every share.
+int asm_batch_autotune(asm_batch* b, const int* leaf_sizes, int n_leaf, int* leaf_out, int* tiles_out)
+{
+    if (!b || b->cubes.empty()) return batch_fail(b, ASM_ERR_STATE, "empty batch");
+    batch_release_graph(b);
+    size_t big = 0;
+    for (size_t i = 1; i < b->cubes.size(); ++i)
+        if (b->cubes[i].n > b->cubes[big].n) big = i;
+    asm_batch::Cube& c = b->cubes[big];
+    int rc = asm_tree_construct(c.ctx, c.left, c.right, nullptr);
+    if (!rc) rc = asm_neighbor_search(c.ctx, kSlots - 1);
+    std::vector<int> tiles(c.ncs.size());
+    int leaf = 0;
+    if (!rc) rc = asm_autotune(c.ctx, leaf_sizes, n_leaf, c.ncs.data(), static_cast<int>(c.ncs.size()), c.sign,
+                               &leaf, tiles.data());
+    if (rc) {
+        b->err = asm_last_error(c.ctx);
+        return rc;
+    }
+    for (auto& o : b->cubes) {
+        if (o.ctx->leaf_size != leaf) {
+            o.ctx->leaf_size = leaf;
+            o.ctx->have_tree = o.ctx->have_search = false;
+        }
+        for (size_t i = 0; i < c.ncs.size(); ++i) o.ctx->tile_for_nc[c.ncs[i]] = tiles[i];
+    }
+    if (leaf_out) *leaf_out = leaf;
+    if (tiles_out)
+        for (size_t i = 0; i < tiles.size(); ++i) tiles_out[i] = tiles[i];
+    return ASM_OK;
+}
+
+// Runs every share once eagerly (sizing all buffers), then records the same
+// work into a graph.  n_streams: how many streams the shares are spread over.
+int asm_batch_capture(asm_batch* b, int n_streams)
+{
+    if (!b || b->cubes.empty()) return batch_fail(b, ASM_ERR_STATE, "empty batch");
+    if (n_streams < 1) return batch_fail(b, ASM_ERR_ARG, "n_streams %d < 1", n_streams);
+    HIPC2(b, hipSetDevice(b->device));
+    batch_release_graph(b);
+    b->total_off.clear();
+    size_t slots = 0;
+    for (auto& c : b->cubes) {
+        b->total_off.push_back(slots);
+        slots += 4 * c.ncs.size();
+    }
+    HIPC2(b, b->totals.ensure(slots));
+    // Eager run: every buffer reaches its size.
+    for (size_t i = 0; i < b->cubes.size(); ++i) {
+        asm_batch::Cube& c = b->cubes[i];
+        c.ctx->profile = 0;
+        const int rc = run_cube(c, b->totals.p + b->total_off[i]);
+        if (rc) {
+            b->err = asm_last_error(c.ctx);
+            return rc;
+        }
+    }
+    for (auto& c : b->cubes) HIPC2(b, hipStreamSynchronize(c.ctx->stream));
+    // Capture: fork from main to the share streams, join back.
+    const int ns = std::min<int>(n_streams, static_cast<int>(b->cubes.size()));
+    std::vector<hipEvent_t> ev(ns + 1);
+    for (auto& e : ev) HIPC2(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    int rc = ASM_OK;
+    HIPC2(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipEventRecord(ev[ns], b->main);
+    // Shares on stream s = i % ns run in order on that stream.
+    std::vector<hipStream_t> lane_stream(ns);
+    for (int s = 0; s < ns && e == hipSuccess; ++s) {
+        lane_stream[s] = b->cubes[s].ctx->stream;
+        e = hipStreamWaitEvent(lane_stream[s], ev[ns], 0);
+    }
+    for (size_t i = 0; i < b->cubes.size() && e == hipSuccess && rc == ASM_OK; ++i) {
+        asm_batch::Cube& c = b->cubes[i];
+        hipStream_t own = c.ctx->stream;
+        c.ctx->stream = lane_stream[i % ns];
+        c.ctx->capturing = true;
+        rc = run_cube(c, b->totals.p + b->total_off[i]);
+        c.ctx->capturing = false;
+        c.ctx->stream = own;
+        if (rc) b->err = asm_last_error(c.ctx);
+    }
+    for (int s = 0; s < ns && e == hipSuccess; ++s) {
+        e = hipEventRecord(ev[s], lane_stream[s]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->main, ev[s], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->main, &g);
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return batch_fail(b, ASM_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    HIPC2(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return ASM_OK;
+}
+
+// The same work without the graph: every share in order, each on its own
+// context stream, waiting for each before the next (the one-share-at-a-time
+// baseline of the batched configuration).
+int asm_batch_run_sequential(asm_batch* b)
+{
+    if (!b || b->cubes.empty() || b->total_off.size() != b->cubes.size())
+        return batch_fail(b, ASM_ERR_STATE, "asm_batch_run_sequential before asm_batch_capture");
+    HIPC2(b, hipSetDevice(b->device));
+    for (size_t i = 0; i < b->cubes.size(); ++i) {
+        asm_batch::Cube& c = b->cubes[i];
+        const int rc = run_cube(c, b->totals.p + b->total_off[i]);
+        if (rc) {
+            b->err = asm_last_error(c.ctx);
+            return rc;
+        }
+        HIPC2(b, hipStreamSynchronize(c.ctx->stream));
+    }
+    return ASM_OK;
+}
+
+int asm_batch_launch(asm_batch* b)
+{
+    if (!b || !b->exec) return batch_fail(b, ASM_ERR_STATE, "asm_batch_launch before asm_batch_capture");
+    HIPC2(b, hipSetDevice(b->device));
+    HIPC2(b, hipGraphLaunch(b->exec, b->main));
+    return ASM_OK;
+}
+
+int asm_batch_synchronize(asm_batch* b)
+{
+    if (!b) return ASM_ERR_ARG;
+    HIPC2(b, hipSetDevice(b->device));
+    HIPC2(b, hipStreamSynchronize(b->main));
+    return ASM_OK;
+}
+
+int asm_batch_size(const asm_batch* b)
+{
+    return b ? static_cast<int>(b->cubes.size()) : 0;
+}
+
+// Normalised mesh (6 nc^3 floats) and raw FP64 totals {sum n, sum halo,
+// sum n^2, sum c} of share i after the last launch.
+int asm_batch_get_mesh(asm_batch* b, int i, int nc, float* mesh, double totals[4])
+{
+    if (!b || i < 0 || i >= static_cast<int>(b->cubes.size())) return batch_fail(b, ASM_ERR_ARG, "no share %d", i);
+    asm_batch::Cube& c = b->cubes[i];
+    size_t k = 0;
+    while (k < c.ncs.size() && c.ncs[k] != nc) ++k;
+    if (k == c.ncs.size()) return batch_fail(b, ASM_ERR_ARG, "share %d has no mesh nc %d", i, nc);
+    HIPC2(b, hipSetDevice(b->device));
+    HIPC2(b, hipStreamSynchronize(b->main));
+    if (mesh) {
+        const int rc = asm_get_mesh(c.ctx, nc, mesh);
+        if (rc) {
+            b->err = asm_last_error(c.ctx);
+            return rc;
+        }
+    }
+    if (totals && !b->total_off.empty())
+        HIPC2(b, hipMemcpy(totals, b->totals.p + b->total_off[i] + 4 * k, sizeof(double) * 4, hipMemcpyDeviceToHost));
+    return ASM_OK;
+}
+
+// h and density of share i (caller order) after the last launch.
+int asm_batch_get_rk_dens(asm_batch* b, int i, float* rk, float* dens)
+{
+    if (!b || i < 0 || i >= static_cast<int>(b->cubes.size())) return batch_fail(b, ASM_ERR_ARG, "no share %d", i);
+    HIPC2(b, hipSetDevice(b->device));
+    HIPC2(b, hipStreamSynchronize(b->main));
+    const int rc = asm_get_rk_dens(b->cubes[i].ctx, rk, dens, nullptr);
+    if (rc) b->err = asm_last_error(b->cubes[i].ctx);
+    return rc;
 }
 
 }  // extern "C"

@@ -213,7 +213,7 @@ __device__ __forceinline__ void tile_pass(const DepositArgs& a, uint32_t* counte
             t = (axis_at(tz, kz) * a.ntx + axis_at(ty, ky)) * a.ntx + axis_at(tx, kx);
         }
         const uint32_t slot = wave_aggregated_inc(counters, t, act);
-        if (list && act) list[tile_start[t] + slot] = static_cast<uint32_t>(i);
+        if (list && act && tile_start[t] + slot < a.list_cap) list[tile_start[t] + slot] = static_cast<uint32_t>(i);
     }
 }
 
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
 
     const float dxc = a.boxsize / static_cast<float>(a.nc);
     const uint32_t begin = tile_start[tile];
-    const uint32_t count = tile_cnt[tile];
+    const uint32_t count = min(tile_cnt[tile], begin < a.list_cap ? a.list_cap - begin : 0u);
     // Lanes of a wavefront take list entries `stride` apart: the list is in
     // Morton order, so neighbouring entries share cells, and LDS atomics of
     // one instruction to the same address would serialise.
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
 
     const float dxc = a.boxsize / static_cast<float>(a.nc);
     const uint32_t begin = tile_start[tile];
-    const uint32_t count = tile_cnt[tile];
+    const uint32_t count = min(tile_cnt[tile], begin < a.list_cap ? a.list_cap - begin : 0u);
     const uint32_t nb = (count + kWave - 1) / kWave;   // batches; batch b holds entries l * nb + b
     int tag = 0;                                       // sweep counter, tags the map marks
     for (uint32_t b = wid; b < nb; b += kDepWaves) {

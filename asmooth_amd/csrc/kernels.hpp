@@ -121,6 +121,7 @@ struct DepositArgs {
     int ntx;         // tiles per axis
     int periodic;    // 0: node mesh (coarse_mesh.cpp), 1: periodic variant (global_mesh.cpp)
     int coord[3];    // node coordinate for the periodic variant
+    uint32_t list_cap;   // entries the tile list holds (writes and reads past it are dropped)
 };
 hipError_t launch_deposit_count(const DepositArgs& a, uint32_t* tile_cnt, hipStream_t s);
 hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start, uint32_t* tile_cur,

@@ -67,6 +67,82 @@ def run_step(ctx, L: float, bf: float, ncs):
     return totals
 
 
+def learned_seed_leg(ctx, parts, L: float, bf: float, ncs, steps: int, plain_ms: float, plain_rk, margin: float):
+    """The same workload with the learned seed radius (asmooth_amd/seed_mlp.npy,
+    trained on Zel'dovich sets of seeds 101-108; this workload's seed 1 is
+    held out): per-step time, fallback rate, and h compared bit for bit."""
+    import asmooth_amd as A
+    from asmooth_amd.learned_seed import DEFAULT_MODEL
+    if not os.path.exists(DEFAULT_MODEL):
+        return None
+    model = A.Mlp(0, np.load(DEFAULT_MODEL))
+    ctx.set_seed_model(model, margin)
+    run_step(ctx, L, bf, ncs)
+    ctx.synchronize()
+    ctx.reset_timers()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run_step(ctx, L, bf, ncs)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    st = ctx.seed_stats()
+    rk, _, _ = ctx.get_rk_dens()
+    timers = ctx.timers()
+    ctx.set_seed_model(None)
+    model.close()
+    nl = ctx.np_local
+    return {"value": round(nl / (ms * 1e-3), 1), "unit": "particles/s", "ms_per_step": round(ms, 3),
+            "gain_vs_plain": round(plain_ms / ms, 4), "margin": margin,
+            "fallback_rate": round(st["fallback_particles"] / st["particles"], 6),
+            "fallback_wavefront_rate": round(st["fallback_wavefronts"] * 64 / st["particles"], 6),
+            "h_bit_exact_vs_plain": bool(np.array_equal(rk.view(np.int32), plain_rk.view(np.int32))),
+            "stage_ms": {k: round(v[0] / steps, 4) for k, v in timers.items() if v[1]}}
+
+
+def batched_leg(steps: int, n_streams: int = 8):
+    """64 independent periodic cubes of side 128 (seeds 1000-1063), N_i = n_i^3
+    with ln N_i ~ N(ln 2^18, ln 2^2) clipped to [2^15, 2^21], Zel'dovich rms
+    U[1, 4], buffer 6 units, nc = 32, 64, 128 each: one recorded HIP graph
+    per step, against the same shares run one after another."""
+    import asmooth_amd as A
+    from asmooth_amd import synth
+    L, bf, ncs = 128.0, 6.0 / 128.0, [32, 64, 128]
+    t = time.time()
+    cubes = synth.batched_cubes(n_cubes=64, L=L, seed0=1000)
+    b = A.Batch(0)
+    n_local = n_buf = 0
+    for xv in cubes:
+        parts, nl = A.build_buffered_set(xv, L, bf)
+        b.add(parts, nl, L, -bf * L, (1 + bf) * L, ncs, -1)
+        n_local += nl
+        n_buf += len(parts)
+    log(f"[bench] batched: {len(cubes)} shares, {n_local} local / {n_buf} buffered particles "
+        f"({time.time() - t:.1f} s to generate)")
+    leaf, tiles = b.autotune((16, 24, 32))
+    b.capture(n_streams)
+    b.launch()
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch()
+    b.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    b.run_sequential()
+    seq_ms = (time.perf_counter() - t0) * 1e3
+    ok = True
+    for i in range(len(cubes)):
+        for nc in ncs:
+            _, tot = b.get_mesh(i, nc)
+            ok = ok and bool(0.999 < (tot[0] + tot[1]) / b.shares[i]["np_local"] < 1.001)
+    b.close()
+    log(f"[bench] batched: {ms:.2f} ms per graph launch, {seq_ms:.2f} ms one share after another")
+    return {"value": round(n_local / (ms * 1e-3), 1), "unit": "particles/s", "ms_per_step": round(ms, 3),
+            "sequential_ms": round(seq_ms, 3), "gain_vs_sequential": round(seq_ms / ms, 4),
+            "shares": len(cubes), "n_local": n_local, "n_buf": n_buf, "ncs": ncs, "streams": n_streams,
+            "autotuned": {"leaf_size": leaf, "tiles": dict(zip(map(str, ncs), tiles))}, "mass_check_ok": ok}
+
+
 def cpu_baseline(xv: np.ndarray, L: float, bf: float, ncs, nn: int = 2):
     """The reference's construct / neighbour loop / assign_on_mesh (compiled from
     its sources, oracle/_ref) on one node share of an nn^3 decomposition of the
@@ -104,6 +180,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-side", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--seed-margin", type=float, default=0.3)
     ap.add_argument("--leaf-size", type=int, default=32, help="particles per tree leaf (8..64)")
     ap.add_argument("--tile", type=int, default=8, help="deposition tile side in cells (1..8)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -175,6 +254,18 @@ def main(argv=None) -> int:
 
     b_alg = 80 * nbuf + 24 * sum(nc ** 3 for nc in ncs)
     mass_ok = all(0.999 < (t[0] + t[1]) / nl < 1.001 for t in totals)
+    learned = None
+    if not args.no_learned:
+        plain_rk, _, _ = ctx.get_rk_dens()
+        learned = learned_seed_leg(ctx, parts, L, bf, ncs, max(2, min(args.steps, 3)), ms_per_step, plain_rk,
+                                   args.seed_margin)
+        if learned:
+            log(f"[bench] learned seed: {learned['ms_per_step']} ms/step, gain {learned['gain_vs_plain']}, "
+                f"fallback {learned['fallback_rate']}, h bit-exact {learned['h_bit_exact_vs_plain']}")
+    ctx.close()
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(max(2, min(args.steps, 5)))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(xv, L, bf, ncs)
@@ -194,8 +285,9 @@ def main(argv=None) -> int:
                    "search_stats_per_query": {k: round(v / nbuf, 2) for k, v in stats.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_seed": learned,
+        "batched": batched,
     }
-    ctx.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -232,6 +232,36 @@ int asm_set_seed_model(asm_context* ctx, const asm_mlp* m, float margin);
  * out[1] wavefronts (64 sorted particles) searched again, out[2] particles. */
 int asm_get_seed_stats(asm_context* ctx, int64_t out[3]);
 
+/* Batches of independent node shares (no counterpart in the reference,
+ * whose ranks each run one share).  Each share gets its own context; its tree
+ * build, search, the depositions of its nc list, the FP64 mesh totals and
+ * the normalisation are recorded once into one HIP graph (shares spread over
+ * n_streams streams forked from one launch stream), which is then launched
+ * as a whole.  asm_batch_add returns the share's index. */
+typedef struct asm_batch asm_batch;
+int asm_batch_create(asm_batch** b, int device);
+void asm_batch_destroy(asm_batch* b);
+const char* asm_batch_last_error(const asm_batch* b);
+int asm_batch_add(asm_batch* b, const asm_particle* particles, int64_t n, int64_t np_local, float boxsize,
+                  float left, float right, const int* ncs, int n_nc, int sign);
+int asm_batch_size(const asm_batch* b);
+/* asm_set_option on every share. */
+int asm_batch_set_option(asm_batch* b, int option, int value);
+/* asm_autotune on the share with the most particles; its choice is used by
+ * every share.  tiles_out: one per nc of that share. */
+int asm_batch_autotune(asm_batch* b, const int* leaf_sizes, int n_leaf, int* leaf_out, int* tiles_out);
+/* Runs every share once (sizing its buffers) and records the graph. */
+int asm_batch_capture(asm_batch* b, int n_streams);
+int asm_batch_launch(asm_batch* b);
+int asm_batch_synchronize(asm_batch* b);
+/* The same work without the graph, one share after another (a baseline). */
+int asm_batch_run_sequential(asm_batch* b);
+/* Normalised mesh (6 nc^3 floats) and raw FP64 totals {sum n, sum halo,
+ * sum n^2, sum c} of share i after the last launch (either may be NULL). */
+int asm_batch_get_mesh(asm_batch* b, int i, int nc, float* mesh, double totals[4]);
+/* h and density of share i, in its particles' order. */
+int asm_batch_get_rk_dens(asm_batch* b, int i, float* rk, float* dens);
+
 /* Timers (HIP events on the context stream) when ASM_OPT_PROFILE is set. */
 int asm_timer_count(void);
 const char* asm_timer_name(int i);
