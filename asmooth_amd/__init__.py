@@ -47,6 +47,7 @@ OPT_LEAF_SIZE = 3
 OPT_TILE = 4
 OPT_DETERMINISTIC = 5
 OPT_KNN_METHOD = 6
+OPT_DEPOSIT_ACCUM = 7
 
 
 class AsmoothError(RuntimeError):

@@ -52,10 +52,14 @@ enum TimerId {
     T_NORMALIZE,
     T_HALO,
     T_SEED,
+    T_KNN_BOUND,
+    T_KNN_COLLECT,
+    T_KNN_FIXUP,
     T_COUNT
 };
-const char* kTimerNames[T_COUNT] = {"sort", "tree", "knn", "dens", "deposit_lists", "deposit_tiles",
-                                    "mesh_reduce", "normalize", "mark_halo", "seed"};
+const char* kTimerNames[T_COUNT] = {"sort",      "tree",      "knn",       "dens",        "deposit_lists",
+                                    "deposit_tiles", "mesh_reduce", "normalize", "mark_halo", "seed",
+                                    "knn_bound", "knn_collect", "knn_fixup"};
 
 struct MeshDev {
     DevBuf<float> mesh;
@@ -99,6 +103,7 @@ struct asm_context {
     DevBuf<double> dens64;
     int deterministic = 0;
     int knn_method = kKnnBoundCollect;
+    int deposit_accum = kAccumFp64;
     DevBuf<uint32_t> knn_flags, knn_leaf_n, knn_leaves;   // per wavefront of 64 sorted particles
     DevBuf<float> knn_tau;                                // per sorted particle
     DevBuf<double> dens_pull;
@@ -186,6 +191,14 @@ struct StageTimer {
     }
 };
 
+// KnnMarks callbacks: one StageTimer-like event pair per kNN kernel.
+struct KnnMarkState {
+    asm_context* c;
+    hipEvent_t a[3] = {nullptr, nullptr, nullptr};
+};
+void knn_mark_begin(void* user, int st);
+void knn_mark_end(void* user, int st);
+
 hipError_t resolve_timers(asm_context* c)
 {
     if (c->pending.empty()) return hipSuccess;
@@ -202,6 +215,24 @@ hipError_t resolve_timers(asm_context* c)
     }
     c->pending.clear();
     return hipSuccess;
+}
+
+void knn_mark_begin(void* user, int st)
+{
+    auto* m = static_cast<KnnMarkState*>(user);
+    if (!m->c->profile) return;
+    m->a[st] = get_event(m->c);
+    if (m->a[st]) (void)hipEventRecord(m->a[st], m->c->stream);
+}
+
+void knn_mark_end(void* user, int st)
+{
+    auto* m = static_cast<KnnMarkState*>(user);
+    if (!m->c->profile || !m->a[st]) return;
+    hipEvent_t b = get_event(m->c);
+    if (!b) return;
+    (void)hipEventRecord(b, m->c->stream);
+    m->c->pending.push_back({T_KNN_BOUND + st, m->a[st], b});
 }
 
 int64_t next_pow2(int64_t v)
@@ -287,6 +318,11 @@ int asm_set_option(asm_context* ctx, int option, int value)
         if (value != kKnnBoundCollect && value != kKnnInsertion)
             return fail(ctx, ASM_ERR_ARG, "kNN method %d is not 0 or 1", value);
         ctx->knn_method = value;
+        return ASM_OK;
+    case ASM_OPT_DEPOSIT_ACCUM:
+        if (value != kAccumFp64 && value != kAccumKahan32 && value != kAccumAtomic32)
+            return fail(ctx, ASM_ERR_ARG, "deposition accumulation %d is not 0, 1 or 2", value);
+        ctx->deposit_accum = value;
         return ASM_OK;
     case ASM_OPT_TILE:
         if (value < 1 || value > 8) return fail(ctx, ASM_ERR_ARG, "tile %d outside 1..8", value);
@@ -426,9 +462,11 @@ int asm_neighbor_search(asm_context* ctx, int k)
         StageTimer t(ctx, T_KNN);
         HIPC(launch_fill_zero_u32(reinterpret_cast<uint32_t*>(ctx->kstats.p), kStatSlots * sizeof(KnnStats) / 4, s));
         if (!ctx->deterministic) HIPC(launch_fill_zero_f64(ctx->dens64.p, n, s));
+        KnnMarkState ms{ctx};
+        const KnnMarks marks{&ms, knn_mark_begin, knn_mark_end};
         HIPC(launch_knn(ctx->spos.p, n, ctx->nodes.p, ctx->nleaf_pad, ctx->nleaf, ctx->leaf_size, ctx->rk.p,
                         ctx->kkey.p, ctx->inv_perm.p, ctx->deterministic ? nullptr : ctx->dens64.p, nbr, ctx->kstats.p,
-                        kscr, ctx->knn_method, s));
+                        kscr, ctx->knn_method, s, &marks));
     }
     {
         StageTimer t(ctx, T_DENS);
@@ -504,6 +542,7 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     }
     a.ntx = (nc + a.tile - 1) / a.tile;
     a.periodic = periodic;
+    a.accum = ctx->deposit_accum;
     a.coord[0] = coord ? coord[0] : 0;
     a.coord[1] = coord ? coord[1] : 0;
     a.coord[2] = coord ? coord[2] : 0;

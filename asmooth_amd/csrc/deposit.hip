@@ -228,8 +228,9 @@ __global__ void deposit_fill_kernel(DepositArgs a, const uint32_t* __restrict__ 
     tile_pass(a, tile_cur, tile_start, list);
 }
 
-// Write one tile's FP64 accumulators to the mesh (FP32), every cell once.
-__device__ __forceinline__ void flush_tile(const DepositArgs& a, const double (*acc)[kTileCells], int cx0, int cy0,
+// Write one tile's accumulators to the mesh (FP32), every cell once.
+template <typename Acc>
+__device__ __forceinline__ void flush_tile(const DepositArgs& a, const Acc (*acc)[kTileCells], int cx0, int cy0,
                                            int cz0, float* __restrict__ mesh, int accumulate)
 {
     const int tid = threadIdx.x;
@@ -237,7 +238,7 @@ __device__ __forceinline__ void flush_tile(const DepositArgs& a, const double (*
     const int ncell = T * T * T;
     const int64_t nc = a.nc;
     const int64_t nmesh = nc * nc * nc;
-    for (int c = tid; c < ncell; c += kDepThreads) {
+    for (int c = tid; c < ncell; c += static_cast<int>(blockDim.x)) {
         const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
         const int gx = cx0 + lx, gy = cy0 + ly, gz = cz0 + lz;
         if (gx >= a.nc || gy >= a.nc || gz >= a.nc) continue;
@@ -369,11 +370,12 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
 constexpr int kDepWaves = kDepThreads / kWave;
 constexpr int kWStride = 3 * kTileMax + 1;   // odd row stride: entries fall in different banks
 
+template <typename Acc>
 __global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
     DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
     const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
 {
-    __shared__ double acc[6][kTileCells];
+    __shared__ Acc acc[6][kTileCells];
     __shared__ float wts[kDepWaves][kWave * kWStride];
     __shared__ int smap[kDepWaves][kWave];
     const int tid = threadIdx.x;
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
     const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
     const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
     const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
-    for (int c = tid; c < 6 * kTileCells; c += kDepThreads) (&acc[0][0])[c] = 0.0;
+    for (int c = tid; c < 6 * kTileCells; c += kDepThreads) (&acc[0][0])[c] = Acc(0);
     map[lane] = -1;
     __syncthreads();
 
@@ -473,13 +475,13 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
                 ker3 = ker3 * wp[2 * kTileMax + iz];
                 const int c = ((bz + iz) * T + (by + iy)) * T + bx + ix;
                 if (igm) {
-                    atomicAdd(&acc[0][c], static_cast<double>(ker3));
-                    atomicAdd(&acc[2][c], static_cast<double>(ker3 * p_dens));
-                    atomicAdd(&acc[3][c], static_cast<double>(ker3 * p_vx));
-                    atomicAdd(&acc[4][c], static_cast<double>(ker3 * p_vy));
-                    atomicAdd(&acc[5][c], static_cast<double>(ker3 * p_vz));
+                    atomicAdd(&acc[0][c], static_cast<Acc>(ker3));
+                    atomicAdd(&acc[2][c], static_cast<Acc>(ker3 * p_dens));
+                    atomicAdd(&acc[3][c], static_cast<Acc>(ker3 * p_vx));
+                    atomicAdd(&acc[4][c], static_cast<Acc>(ker3 * p_vy));
+                    atomicAdd(&acc[5][c], static_cast<Acc>(ker3 * p_vz));
                 } else {
-                    atomicAdd(&acc[1][c], static_cast<double>(ker3));
+                    atomicAdd(&acc[1][c], static_cast<Acc>(ker3));
                 }
             }
         }
@@ -487,6 +489,96 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
     }
     __syncthreads();
     flush_tile(a, acc, cx0, cy0, cz0, mesh, accumulate);
+}
+
+// Precision study: the tile's sums in FP32 with Kahan compensation.  One
+// wavefront owns a tile and takes its list entries one after another, each
+// entry's box of cells spread over the lanes, so that no two updates of one
+// instruction touch the same cell and the compensated update (sum, carry)
+// needs no atomics.  The stored cell is sum - carry.
+constexpr int kKahanThreads = kWave;
+
+__global__ __launch_bounds__(kKahanThreads) void deposit_tile_kahan_kernel(
+    DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
+    const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
+{
+    __shared__ float sum[6][kTileCells];
+    __shared__ float carry[6][kTileCells];
+    __shared__ float wts[3 * kTileMax];
+    const int lane = threadIdx.x;
+    const int T = a.tile;
+    const int64_t tile = blockIdx.x;
+    const int tx = static_cast<int>(tile % a.ntx);
+    const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
+    const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
+    const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
+    for (int c = lane; c < 6 * kTileCells; c += kKahanThreads) {
+        (&sum[0][0])[c] = 0.0f;
+        (&carry[0][0])[c] = 0.0f;
+    }
+    const float dxc = a.boxsize / static_cast<float>(a.nc);
+    const uint32_t begin = tile_start[tile];
+    const uint32_t count = min(tile_cnt[tile], begin < a.list_cap ? a.list_cap - begin : 0u);
+    for (uint32_t e = 0; e < count; ++e) {
+        const int64_t i = list[begin + e];
+        const PartRange r = part_range(a, i);
+        const float hinv = 1.0f / r.h;
+        const float ds = dxc * hinv;
+        const float4 at = a.sattr[i];
+        const float dens = a.dens[i];
+        const bool igm = (a.sign * __float_as_int(at.w) <= 0);
+        const int c0[3] = {cx0, cy0, cz0};
+        int b0[3], n3[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int lo = max(r.lo[d], c0[d]);
+            const int hi = min(r.hi[d], c0[d] + T);
+            b0[d] = lo - c0[d];
+            n3[d] = hi > lo ? hi - lo : 0;
+        }
+        __syncthreads();   // previous entry's weights are no longer read
+        if (lane < 3 * kTileMax) {
+            const int d = lane / kTileMax, t = lane % kTileMax;
+            if (t < n3[d]) {
+                const float s1 = (static_cast<float>(c0[d] + b0[d] + t) * dxc - r.x[d]) * hinv;
+                wts[lane] = ker_integ(s1, s1 + ds);
+            }
+        }
+        __syncthreads();
+        const int nxy = n3[0] * n3[1];
+        const int size = nxy * n3[2];
+        for (int off = lane; off < size; off += kKahanThreads) {
+            const int iz = off / nxy, rem = off % nxy, iy = rem / n3[0], ix = rem % n3[0];
+            float ker3 = wts[ix] * wts[kTileMax + iy];   // coarse_mesh.cpp:93
+            ker3 = ker3 * wts[2 * kTileMax + iz];
+            const int c = ((b0[2] + iz) * T + (b0[1] + iy)) * T + b0[0] + ix;
+            float terms[6];
+            int fields[6];
+            int nf = 0;
+            if (igm) {
+                terms[0] = ker3; fields[0] = 0;
+                terms[1] = ker3 * dens; fields[1] = 2;
+                terms[2] = ker3 * at.x; fields[2] = 3;
+                terms[3] = ker3 * at.y; fields[3] = 4;
+                terms[4] = ker3 * at.z; fields[4] = 5;
+                nf = 5;
+            } else {
+                terms[0] = ker3; fields[0] = 1;
+                nf = 1;
+            }
+            for (int f = 0; f < nf; ++f) {
+                const int fi = fields[f];
+                const float y = terms[f] - carry[fi][c];
+                const float t = sum[fi][c] + y;
+                carry[fi][c] = (t - sum[fi][c]) - y;
+                sum[fi][c] = t;
+            }
+        }
+    }
+    __syncthreads();
+    for (int c = lane; c < 6 * kTileCells; c += kKahanThreads) (&sum[0][0])[c] -= (&carry[0][0])[c];
+    __syncthreads();
+    flush_tile(a, sum, cx0, cy0, cz0, mesh, accumulate);
 }
 
 template <typename T>
@@ -598,9 +690,15 @@ hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start
     if (a.periodic)
         deposit_tile_kernel<<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(a, tile_start, tile_cnt, list,
                                                                                   mesh, accumulate);
+    else if (a.accum == kAccumKahan32)
+        deposit_tile_kahan_kernel<<<static_cast<unsigned>(ntiles), kKahanThreads, 0, s>>>(a, tile_start, tile_cnt,
+                                                                                          list, mesh, accumulate);
+    else if (a.accum == kAccumAtomic32)
+        deposit_tile_packed_kernel<float><<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(
+            a, tile_start, tile_cnt, list, mesh, accumulate);
     else
-        deposit_tile_packed_kernel<<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(a, tile_start, tile_cnt,
-                                                                                         list, mesh, accumulate);
+        deposit_tile_packed_kernel<double><<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(
+            a, tile_start, tile_cnt, list, mesh, accumulate);
     return hipGetLastError();
 }
 

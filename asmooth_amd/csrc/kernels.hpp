@@ -72,9 +72,17 @@ struct KnnScratch {
     uint32_t* seed_redo;
     unsigned long long* seed_fallbacks;
 };
+// Optional marks around the kernels of the bound/collect method (stage 0:
+// bound walk, 1: collect walk, 2: insertion kernel for flagged wavefronts).
+struct KnnMarks {
+    void* user;
+    void (*begin)(void* user, int stage);
+    void (*end)(void* user, int stage);
+};
 hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
                       int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, const int32_t* inv_perm, double* dens64,
-                      int32_t* nbr_out_orig, KnnStats* stats, const KnnScratch& scratch, int method, hipStream_t s);
+                      int32_t* nbr_out_orig, KnnStats* stats, const KnnScratch& scratch, int method, hipStream_t s,
+                      const KnnMarks* marks = nullptr);
 // dens = dens64 (+ dens_pull when non-null), rounded to FP32.
 hipError_t launch_dens_finalize(const double* dens64, const double* dens_pull, int64_t n, float* dens, hipStream_t s);
 // Per-node maximum of the 33rd squared distance (2*nleaf_pad floats).
@@ -122,7 +130,11 @@ struct DepositArgs {
     int periodic;    // 0: node mesh (coarse_mesh.cpp), 1: periodic variant (global_mesh.cpp)
     int coord[3];    // node coordinate for the periodic variant
     uint32_t list_cap;   // entries the tile list holds (writes and reads past it are dropped)
+    int accum;           // kAccumFp64 (default), kAccumKahan32, kAccumAtomic32 (node mesh only)
 };
+constexpr int kAccumFp64 = 0;      // FP64 LDS atomics
+constexpr int kAccumKahan32 = 1;   // FP32 with Kahan compensation, one wavefront per tile
+constexpr int kAccumAtomic32 = 2;  // plain FP32 LDS atomics
 hipError_t launch_deposit_count(const DepositArgs& a, uint32_t* tile_cnt, hipStream_t s);
 hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start, uint32_t* tile_cur,
                                uint32_t* list, hipStream_t s);

@@ -798,9 +798,16 @@ hipError_t launch_dens_finalize(const double* dens64, const double* dens_pull, i
 
 hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t nleaf_pad, int64_t nleaf,
                       int leaf_size, float* rk_sorted, uint64_t* kkey_sorted, const int32_t* inv_perm, double* dens64,
-                      int32_t* nbr_out_orig, KnnStats* stats, const KnnScratch& scratch, int method, hipStream_t s)
+                      int32_t* nbr_out_orig, KnnStats* stats, const KnnScratch& scratch, int method, hipStream_t s,
+                      const KnnMarks* marks)
 {
     if (n <= 0) return hipSuccess;
+    auto mark_begin = [&](int st) {
+        if (marks && marks->begin) marks->begin(marks->user, st);
+    };
+    auto mark_end = [&](int st) {
+        if (marks && marks->end) marks->end(marks->user, st);
+    };
     const int64_t waves = (n + kWave - 1) / kWave;
     const int init_leaves = leaf_size >= kWave ? 1 : kWave / leaf_size;
     const unsigned ins_blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
@@ -812,6 +819,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     }
     if (leaf_size < 1 || leaf_size > kWave) return hipErrorInvalidValue;
     const unsigned bound_blocks = static_cast<unsigned>((waves + kBoundWaves - 1) / kBoundWaves);
+    mark_begin(0);
     knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n, stats,
         scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks);
@@ -823,6 +831,8 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
             stats, nullptr, scratch.seed_redo, nullptr, nullptr);
         ASM_HIP_CHECK(hipGetLastError());
     }
+    mark_end(0);
+    mark_begin(1);
     const unsigned col_blocks = static_cast<unsigned>((waves + kCollectWaves - 1) / kCollectWaves);
 #define ASM_COLLECT(M, PAD, ALL)                                                                                \
     knn_collect_kernel<kSlots, M, PAD, ALL><<<col_blocks, kCollectThreads, 0, s>>>(                             \
@@ -841,12 +851,16 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
 #undef ASM_COLLECT_PAD
 #undef ASM_COLLECT
     ASM_HIP_CHECK(hipGetLastError());
+    mark_end(1);
     // Wavefronts flagged by the collect walk are redone by the insertion
     // kernel; every other wavefront returns at once.
+    mark_begin(2);
     knn_dens_kernel<kSlots><<<ins_blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size,
                                                               init_leaves, rk_sorted, kkey_sorted, inv_perm, dens64,
                                                               nbr_out_orig, stats, scratch.wave_flags);
-    return hipGetLastError();
+    ASM_HIP_CHECK(hipGetLastError());
+    mark_end(2);
+    return hipSuccess;
 }
 
 hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t nleaf, int64_t nleaf_pad, float* kmax,

@@ -26,12 +26,15 @@ def _short(name: str) -> str:
     return base.split("::")[-1].replace("void ", "").strip()
 
 
-def summarize(stats_csv: str, fetch_csv: str, write_csv: str, command: str) -> str:
+def summarize(stats_csv: str, fetch_csv: str, write_csv: str, commands: str, json_out: str | None = None) -> str:
     """Markdown summary: per-kernel time from the kernel-trace stats run and
     HBM bytes per dispatch from the FETCH_SIZE / WRITE_SIZE runs, calibrated
-    by the copy kernel (bytes copied / counter reading, per counter)."""
+    by the copy kernel (bytes copied / counter reading, per counter).  With
+    json_out, the same figures as JSON (kernel -> avg_ms, read_bytes,
+    write_bytes)."""
     import csv
     import collections
+    import json
 
     def counters(fn, cname):
         d = collections.defaultdict(list)
@@ -46,25 +49,36 @@ def summarize(stats_csv: str, fetch_csv: str, write_csv: str, command: str) -> s
     w_cal = COPY_BYTES / write["copy_kernel"][0]
     rows = list(csv.DictReader(open(stats_csv)))
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
-    out = [f"Command: `{command}`", "",
-           f"Calibration (copy kernel, {COPY_BYTES} bytes in and out): FETCH_SIZE x {f_cal:.3f}, "
-           f"WRITE_SIZE x {w_cal:.3f}.", "",
-           "| kernel | calls | avg ms | total % | HBM read MB/launch | HBM write MB/launch |",
-           "|---|---:|---:|---:|---:|---:|"]
+    out = ["Commands (from the repository root, TMPDIR=/tmp):", ""]
+    out += [f"    {c}" for c in commands.split(" ;; ")]
+    out += ["", f"Calibration (copy kernel, {COPY_BYTES} bytes in and out): FETCH_SIZE x {f_cal:.3f}, "
+            f"WRITE_SIZE x {w_cal:.3f}.", "",
+            "| kernel | calls | avg ms | total % | HBM read MB/launch | HBM write MB/launch |",
+            "|---|---:|---:|---:|---:|---:|"]
+    js = {"calibration": {"fetch": f_cal, "write": w_cal}, "kernels": {}}
     for r in rows:
         k = _short(r["Name"])
         fr = fetch.get(k)
         wr = write.get(k)
-        rd = f"{sum(fr) / len(fr) * f_cal / 1e6:.1f}" if fr else "-"
-        wt = f"{sum(wr) / len(wr) * w_cal / 1e6:.1f}" if wr else "-"
+        rb = sum(fr) / len(fr) * f_cal if fr else None
+        wb = sum(wr) / len(wr) * w_cal if wr else None
+        rd = f"{rb / 1e6:.1f}" if rb is not None else "-"
+        wt = f"{wb / 1e6:.1f}" if wb is not None else "-"
         out.append(f"| {k} | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | {float(r['Percentage']):.1f} | "
                    f"{rd} | {wt} |")
+        js["kernels"][k] = {"calls": int(r["Calls"]), "avg_ms": float(r["AverageNs"]) / 1e6,
+                            "read_bytes": rb, "write_bytes": wb}
+    if json_out:
+        with open(json_out, "w") as f:
+            json.dump(js, f, indent=1)
     return "\n".join(out) + "\n"
 
 
 def main() -> int:
     if len(sys.argv) > 1 and sys.argv[1] == "summarize":
-        print(summarize(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5]))
+        # summarize <stats.csv> <fetch.csv> <write.csv> "<cmd1> ;; <cmd2> ;; ..." [json_out]
+        print(summarize(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5],
+                        sys.argv[6] if len(sys.argv) > 6 else None))
         return 0
     n_side = int(sys.argv[1]) if len(sys.argv) > 1 else 256
     L, bf = 2.0 * n_side, 0.05

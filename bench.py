@@ -36,12 +36,30 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes (compulsory FP32 traffic) per unit of work:
 BYTES_TREE = 24      # per particle: read x, write permuted x
 BYTES_KNN = 20       # per particle: read x, write h, write dens
+BYTES_KNN_BOUND = 12  # per particle: read x (the bound walk's compulsory input)
 BYTES_DEPOSIT = 36   # per particle per mesh: read x, v, h, dens, igrp
 BYTES_CELL = 24      # per cell per mesh: 6 fields written once
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_zeldovich256.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration) from the committed counter summary of this
+    workload, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    if not k or k.get("read_bytes") is None or k.get("write_bytes") is None:
+        return None
+    return int(k["read_bytes"] + k["write_bytes"])
 
 
 def make_workload(n_side: int, L: float, bf: float, seed: int):
@@ -239,18 +257,25 @@ def main(argv=None) -> int:
         + ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}ms x{v[1]}" for k, v in timers.items() if v[1]))
 
     # Roofline of the dominant kernel (largest total time in the timed steps).
+    # Timers are HIP events on the library's stream around each kernel (the
+    # kNN kernels individually; deposit_tiles once per mesh).
     alg = {
-        "knn": (BYTES_KNN * nbuf, "kNN + h (+ 33rd key for the density gather); 20 B/particle x N_buf"),
+        "knn_bound": (BYTES_KNN_BOUND * nbuf, "knn_bound_kernel",
+                      "bound walk: read x, 12 B/particle x N_buf"),
+        "knn_collect": (BYTES_KNN * nbuf, "knn_collect_kernel",
+                        "collect walk: read x, write h and dens, 20 B/particle x N_buf"),
         "deposit_tiles": (sum(BYTES_DEPOSIT * nbuf + BYTES_CELL * nc ** 3 for nc in ncs) / len(ncs),
+                          "deposit_tile_packed_kernel",
                           "tile deposition; 36 B/particle x N_buf + 24 B/cell x nc^3, mean over the nc list"),
-        "sort": (BYTES_TREE * nbuf, "Morton radix sort; 24 B/particle"),
+        "sort": (BYTES_TREE * nbuf, "rs_scatter", "Morton radix sort; 24 B/particle"),
     }
     dom = max((k for k in alg if timers.get(k, (0, 0))[1]), key=lambda k: timers[k][0])
     dom_ms = timers[dom][0] / timers[dom][1]
     achieved = alg[dom][0] / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "launch_ms": round(dom_ms, 4), "bytes_per_launch": int(alg[dom][0]), "note": alg[dom][1]}
+    traffic = measured_traffic(alg[dom][1])
+    roofline = {"bound": "hbm", "kernel": alg[dom][1], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "launch_ms": round(dom_ms, 4), "bytes_per_launch": int(alg[dom][0]), "note": alg[dom][2]}
 
     b_alg = 80 * nbuf + 24 * sum(nc ** 3 for nc in ncs)
     mass_ok = all(0.999 < (t[0] + t[1]) / nl < 1.001 for t in totals)

@@ -68,8 +68,10 @@ enum {
     ASM_OPT_TILE = 4,           /* deposition tile side in cells, 1..8 (default 8) */
     ASM_OPT_DETERMINISTIC = 5,  /* 1: density by an ordered gather, bitwise reproducible
                                       run to run (default 0: FP64 atomic scatter) */
-    ASM_OPT_KNN_METHOD = 6      /* 0: bound walk + collect walk (default); 1: sorted-key
+    ASM_OPT_KNN_METHOD = 6,     /* 0: bound walk + collect walk (default); 1: sorted-key
                                       insertion lists for every wavefront.  Same results. */
+    ASM_OPT_DEPOSIT_ACCUM = 7   /* node-mesh cell sums: 0 FP64 (default); 1 FP32 with Kahan
+                                      compensation; 2 plain FP32 atomics (precision study) */
 };
 
 const char* asm_version(void);

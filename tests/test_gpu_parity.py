@@ -208,6 +208,29 @@ def test_autotune_keeps_results(ctx):
         ctx.set_option(A.OPT_TILE, 8)
 
 
+@pytest.mark.parametrize("accum,rtol", [(1, 1e-6), (2, 1e-4)])
+def test_fp32_deposition_modes(ctx, accum, rtol):
+    # precision-study modes: FP32 sums with Kahan compensation (1) and plain
+    # FP32 atomics (2), against the FP64 oracle sums of the same FP32 terms;
+    # error relative to the cell's sum of |terms| (n * max|v| for velocities)
+    xv, L, bf = small_set("zeldovich", 16, 13)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    o = oracle_path(buf, L, [8, 16])
+    ctx.set_option(A.OPT_DEPOSIT_ACCUM, accum)
+    try:
+        g = gpu_path(ctx, parts, nl, L, bf, [8, 16], keep_neighbors=False)
+    finally:
+        ctx.set_option(A.OPT_DEPOSIT_ACCUM, 0)
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    for nc in (8, 16):
+        gm, om = mesh_fields(g["meshes"][nc], nc), mesh_fields(o["meshes"][nc], nc)
+        for f in ("n", "halo", "c"):
+            assert rel_err(gm[f], om[f], floor=1e-30) <= rtol, (nc, f)
+        scale = np.repeat(om["n"], 3) * vmax + 1e-30
+        assert np.max(np.abs(gm["v"] - om["v"]) / scale) <= rtol
+
+
 def test_too_few_particles_is_an_error(ctx):
     parts = A.particles_from_xv(np.zeros((10, 6), np.float32))
     with pytest.raises(A.AsmoothError):
