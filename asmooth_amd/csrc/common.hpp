@@ -66,6 +66,19 @@ __device__ __forceinline__ float box_d2(const Box& b, float q0, float q1, float 
     return s;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): workgroup w runs on XCD w % 8.  Returns the logical block for w such
+// that each XCD takes one contiguous range of logical blocks, so blocks that
+// are neighbours in Morton (or tile) order share an L2 instead of fetching
+// the same particles once per XCD.  A bijection of [0, nblocks).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t w, uint32_t nblocks)
+{
+    const uint32_t per = nblocks / kXcds, rem = nblocks % kXcds;
+    const uint32_t x = w % kXcds, k = w / kXcds;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 // Wavefront-wide helpers.
 __device__ __forceinline__ bool wave_any(bool p)
 {

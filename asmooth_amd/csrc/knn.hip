@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     __shared__ float4 stage[kWavesPerBlock][kWave];
     const int lane = threadIdx.x % kWave;
     const int wid = wave_in_block();
-    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid) * kWave;
+    const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock + wid) * kWave;
     if (wave0 >= n) return;  // whole wavefront past the end
     const uint32_t redo = only_flagged ? only_flagged[wave0 / kWave] : kRedoAll;
     if (redo == 0u) return;  // done by the collect walk
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     __shared__ float4 stage[kBoundWaves][kWave];
     const int lane = threadIdx.x % kWave;
     const int wid = wave_in_block();
-    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kBoundWaves + wid) * kWave;
+    const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBoundWaves + wid) * kWave;
     if (wave0 >= n) return;
     const int64_t wave = wave0 / kWave;
     if (only_flagged && only_flagged[wave] == 0u) return;
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
     const int lane = threadIdx.x % kWave;
     const int wid = wave_in_block();
     CollectLds<M>& L = lds[wid];
-    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kCollectWaves + wid) * kWave;
+    const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kCollectWaves + wid) * kWave;
     if (wave0 >= n) return;
     const int64_t wave = wave0 / kWave;
     const uint32_t n_rec = wleaf_n[wave];
@@ -732,7 +732,7 @@ __global__ __launch_bounds__(kKnnThreads) void dens_gather_kernel(
     __shared__ float4 stage_k[kWavesPerBlock][kWave];   // (kkey lo, kkey hi, hinv, -)
     const int lane = threadIdx.x % kWave;
     const int wid = wave_in_block();
-    const int64_t wave0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid) * kWave;
+    const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock + wid) * kWave;
     if (wave0 >= n) return;
     const int64_t q = wave0 + lane;
     const bool valid = q < n;
