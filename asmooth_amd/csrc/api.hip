@@ -96,6 +96,7 @@ struct asm_context {
     DevBuf<uint32_t> perm_sorted;  // sorted position -> original index (points into vals or vals_tmp)
     uint32_t* perm = nullptr;
     DevBuf<Box> nodes;
+    DevBuf<Box> groups;   // boxes of kGroup consecutive sorted particles
     int64_t nleaf = 0, nleaf_pad = 0;
     // search results (sorted order)
     DevBuf<float> rk, dens, kmax;
@@ -398,10 +399,13 @@ int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode
     ctx->nleaf = (n + ctx->leaf_size - 1) / ctx->leaf_size;
     ctx->nleaf_pad = next_pow2(ctx->nleaf);
     HIPC(ctx->nodes.ensure(static_cast<size_t>(2 * ctx->nleaf_pad)));
+    const bool use_groups = ctx->leaf_size % kGroup == 0;
+    if (use_groups) HIPC(ctx->groups.ensure(static_cast<size_t>((n + kGroup - 1) / kGroup)));
     {
         StageTimer t(ctx, T_TREE);
         HIPC(launch_gather_sorted(ctx->input.p, ctx->perm, n, ctx->spos.p, ctx->sattr.p, ctx->inv_perm.p, s));
-        HIPC(launch_build_tree(ctx->spos.p, n, ctx->leaf_size, ctx->nleaf, ctx->nleaf_pad, ctx->nodes.p, s));
+        HIPC(launch_build_tree(ctx->spos.p, n, ctx->leaf_size, ctx->nleaf, ctx->nleaf_pad, ctx->nodes.p,
+                               use_groups ? ctx->groups.p : nullptr, s));
     }
     ctx->have_tree = true;
     ctx->have_search = false;
@@ -432,7 +436,7 @@ int asm_neighbor_search(asm_context* ctx, int k)
         if (!ctx->deterministic) HIPC(ctx->dens_pull.ensure(n));
     }
     KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p,
-                    nullptr, nullptr, nullptr};
+                    nullptr, nullptr, nullptr, ctx->leaf_size % kGroup == 0 ? ctx->groups.p : nullptr};
     ctx->seed_used = false;
     if (bound_collect && ctx->seed_on) {
         HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));

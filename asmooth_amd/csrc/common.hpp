@@ -20,6 +20,11 @@ struct __align__(16) Box {
     int hi_w;
 };
 
+// Particles per group box: the tree also keeps the bounding box of every 8
+// consecutive sorted particles, so a scan of an opened leaf can skip groups
+// that no lane's current bound reaches.
+constexpr int kGroup = 8;
+
 // Key ordering neighbour candidates canonically: squared distance first (the
 // bit pattern of a non-negative float orders like the value), then the
 // particle's original index.

@@ -222,6 +222,116 @@ __global__ void deposit_count_kernel(DepositArgs a, uint32_t* __restrict__ tile_
     tile_pass(a, tile_cnt, nullptr, nullptr);
 }
 
+// Node-mesh lists, aggregated per workgroup.  The 256 particles of a
+// workgroup are consecutive in Morton order, so the tiles they touch lie in a
+// small box of tiles: each (particle, tile) pair is counted in LDS, and one
+// global atomic per touched tile adds the workgroup's count (count pass) or
+// reserves the workgroup's slots of that tile's list (fill pass, then each
+// pair takes its slot with an LDS atomic).  A workgroup whose tile box
+// exceeds kListTable falls back to one global atomic per pair.  Both passes
+// see the same pairs, so the fill uses exactly the counted slots.
+constexpr int kListThreads = 256;
+constexpr int kListTable = 1024;
+
+template <bool kFill>
+__global__ __launch_bounds__(kListThreads) void node_list_kernel(DepositArgs a, uint32_t* __restrict__ counters,
+                                                                 const uint32_t* __restrict__ tile_start,
+                                                                 uint32_t* __restrict__ list)
+{
+    __shared__ uint32_t cnt[kListTable];
+    __shared__ uint32_t base[kFill ? kListTable : 1];
+    __shared__ int bb[6];
+    const int tid = threadIdx.x;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kListThreads + tid;
+    int t0[3] = {0, 0, 0}, t1[3] = {-1, -1, -1};
+    if (i < a.n) {
+        const PartRange r = part_range(a, i);
+        if (r.active) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                t0[d] = r.lo[d] / a.tile;
+                t1[d] = (r.hi[d] - 1) / a.tile;
+            }
+        }
+    }
+    const bool active = t1[0] >= 0;
+    if (tid < 6) bb[tid] = (tid < 3) ? 0x7fffffff : -1;
+    __syncthreads();
+    int v[6];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        v[d] = active ? t0[d] : 0x7fffffff;
+        v[3 + d] = active ? t1[d] : -1;
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            v[d] = min(v[d], __shfl_xor(v[d], o));
+            v[3 + d] = max(v[3 + d], __shfl_xor(v[3 + d], o));
+        }
+    }
+    if (tid % kWave == 0 && v[3] >= 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&bb[d], v[d]);
+            atomicMax(&bb[3 + d], v[3 + d]);
+        }
+    }
+    __syncthreads();
+    if (bb[3] < 0) return;   // no active particle in the workgroup (uniform)
+    const int bx = bb[3] - bb[0] + 1, by = bb[4] - bb[1] + 1, bz = bb[5] - bb[2] + 1;
+    const int64_t vol = static_cast<int64_t>(bx) * by * bz;
+    const int ntx = a.ntx;
+
+    if (vol > kListTable) {
+        // Spread-out workgroup: one global atomic per pair.
+        if (!active) return;
+        for (int tz = t0[2]; tz <= t1[2]; ++tz)
+            for (int ty = t0[1]; ty <= t1[1]; ++ty)
+                for (int tx = t0[0]; tx <= t1[0]; ++tx) {
+                    const int64_t t = (static_cast<int64_t>(tz) * ntx + ty) * ntx + tx;
+                    if (kFill) {
+                        const uint32_t pos = tile_start[t] + atomicAdd(&counters[t], 1u);
+                        if (pos < a.list_cap) list[pos] = static_cast<uint32_t>(i);
+                    } else {
+                        atomicAdd(&counters[t], 1u);
+                    }
+                }
+        return;
+    }
+    for (int k = tid; k < vol; k += kListThreads) cnt[k] = 0u;
+    __syncthreads();
+    if (active)
+        for (int tz = t0[2]; tz <= t1[2]; ++tz)
+            for (int ty = t0[1]; ty <= t1[1]; ++ty)
+                for (int tx = t0[0]; tx <= t1[0]; ++tx)
+                    atomicAdd(&cnt[((tz - bb[2]) * by + (ty - bb[1])) * bx + (tx - bb[0])], 1u);
+    __syncthreads();
+    for (int k = tid; k < vol; k += kListThreads) {
+        const uint32_t c = cnt[k];
+        if (c == 0u) continue;
+        const int tx = bb[0] + k % bx, ty = bb[1] + (k / bx) % by, tz = bb[2] + k / (bx * by);
+        const int64_t t = (static_cast<int64_t>(tz) * ntx + ty) * ntx + tx;
+        if (kFill) {
+            base[k] = tile_start[t] + atomicAdd(&counters[t], c);
+            cnt[k] = 0u;
+        } else {
+            atomicAdd(&counters[t], c);
+        }
+    }
+    if (!kFill) return;
+    __syncthreads();
+    if (active)
+        for (int tz = t0[2]; tz <= t1[2]; ++tz)
+            for (int ty = t0[1]; ty <= t1[1]; ++ty)
+                for (int tx = t0[0]; tx <= t1[0]; ++tx) {
+                    const int k = ((tz - bb[2]) * by + (ty - bb[1])) * bx + (tx - bb[0]);
+                    const uint32_t pos = base[k] + atomicAdd(&cnt[k], 1u);
+                    if (pos < a.list_cap) list[pos] = static_cast<uint32_t>(i);
+                }
+}
+
 __global__ void deposit_fill_kernel(DepositArgs a, const uint32_t* __restrict__ tile_start,
                                     uint32_t* __restrict__ tile_cur, uint32_t* __restrict__ list)
 {
@@ -668,6 +778,11 @@ __global__ void normalize_kernel(float* __restrict__ mesh, int64_t nmesh, float 
 hipError_t launch_deposit_count(const DepositArgs& a, uint32_t* tile_cnt, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
+    if (!a.periodic) {
+        node_list_kernel<false><<<static_cast<unsigned>((a.n + kListThreads - 1) / kListThreads), kListThreads, 0,
+                                  s>>>(a, tile_cnt, nullptr, nullptr);
+        return hipGetLastError();
+    }
     const int th = 256;
     deposit_count_kernel<<<static_cast<unsigned>((a.n + th - 1) / th), th, 0, s>>>(a, tile_cnt);
     return hipGetLastError();
@@ -677,6 +792,11 @@ hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start,
                                uint32_t* list, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
+    if (!a.periodic) {
+        node_list_kernel<true><<<static_cast<unsigned>((a.n + kListThreads - 1) / kListThreads), kListThreads, 0,
+                                 s>>>(a, tile_cur, tile_start, list);
+        return hipGetLastError();
+    }
     const int th = 256;
     deposit_fill_kernel<<<static_cast<unsigned>((a.n + th - 1) / th), th, 0, s>>>(a, tile_start, tile_cur, list);
     return hipGetLastError();

@@ -32,8 +32,11 @@ hipError_t launch_morton_keys(const ParticleRec* p, int64_t n, float left, float
 // inv_perm[orig] = sorted position.
 hipError_t launch_gather_sorted(const ParticleRec* p, const uint32_t* perm, int64_t n,
                                 float4* spos, float4* sattr, int32_t* inv_perm, hipStream_t s);
+// nodes: 2*nleaf_pad boxes (implicit binary tree, leaves at nleaf_pad..);
+// groups: ceil(n / kGroup) boxes of kGroup consecutive sorted particles
+// (lo_w = first particle, hi_w = count), or null.
 hipError_t launch_build_tree(const float4* spos, int64_t n, int leaf_size, int64_t nleaf,
-                             int64_t nleaf_pad, Box* nodes, hipStream_t s);
+                             int64_t nleaf_pad, Box* nodes, Box* groups, hipStream_t s);
 
 // ---- knn.hip ---------------------------------------------------------------
 struct KnnStats {
@@ -71,6 +74,10 @@ struct KnnScratch {
     const float* tau_init;
     uint32_t* seed_redo;
     unsigned long long* seed_fallbacks;
+    // Boxes of kGroup consecutive sorted particles (launch_build_tree), used
+    // by the bound walk to skip groups of an opened leaf; null when the leaf
+    // size is not a multiple of kGroup.
+    const Box* groups;
 };
 // Optional marks around the kernels of the bound/collect method (stage 0:
 // bound walk, 1: collect walk, 2: insertion kernel for flagged wavefronts).

@@ -273,37 +273,78 @@ __device__ __forceinline__ void med3_insert(float (&t)[K], float d)
     t[0] = __builtin_amdgcn_fmed3f(0.0f, d, t[0]);
 }
 
-// Stage up to 64 particles of [start, start + cnt) in LDS; slots past the
+// Stage up to 64 particles of [first, first + left) in LDS; slots past the
 // end hold a point at infinity (squared distance +inf: never accepted), so
-// the scan loops can take four candidates per step.
+// the scan loops can take whole groups of candidates.  With group boxes,
+// lanes 0..7 also stage the boxes of the 8 groups (first is a multiple of
+// kGroup); groups past the end get an empty box.  Returns the number of
+// slots to scan, rounded up to whole groups.
 __device__ __forceinline__ int stage_batch(const float4* __restrict__ spos, int64_t first, int64_t left,
-                                           float4* stage, int lane)
+                                           float4* stage, const Box* __restrict__ groups, Box* gstage, int lane)
 {
     const int m = static_cast<int>(left < kWave ? left : kWave);
     stage[lane] = (lane < m) ? spos[first + lane] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    if (groups && lane < kWave / kGroup) {
+        Box b;
+        if (lane * kGroup < m) {
+            b = groups[first / kGroup + lane];
+        } else {
+            b.lo[0] = b.lo[1] = b.lo[2] = INFINITY;
+            b.hi[0] = b.hi[1] = b.hi[2] = -INFINITY;
+            b.lo_w = b.hi_w = 0;
+        }
+        gstage[lane] = b;
+    }
     wave_lds_fence();
-    return (m + 3) & ~3;
+    return (m + kGroup - 1) & ~(kGroup - 1);
 }
 
 template <int K>
 __device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int64_t start, int64_t cnt,
-                                           float4* stage, int lane, float qx, float qy, float qz, float (&t)[K],
-                                           uint32_t& n_acc)
+                                           float4* stage, const Box* __restrict__ groups, Box* gstage, int lane,
+                                           float qx, float qy, float qz, float (&t)[K], uint32_t& n_acc,
+                                           unsigned long long& n_dist)
 {
     for (int64_t base = 0; base < cnt; base += kWave) {
-        const int m4 = stage_batch(spos, start + base, cnt - base, stage, lane);
-        for (int j = 0; j < m4; j += 4) {
-            float d[4];
+        const int m8 = stage_batch(spos, start + base, cnt - base, stage, groups, gstage, lane);
+        for (int g = 0; g < m8; g += kGroup) {
+            // Skip a group whose box no lane's current bound reaches: every
+            // candidate in it is at least the box distance away.
+            if (groups && !wave_any(box_d2(gstage[g / kGroup], qx, qy, qz) <= t[K - 1])) continue;
+            n_dist += kGroup;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float4 c = stage[j + u];
-                d[u] = norm2(qx, qy, qz, c.x, c.y, c.z);
-            }
+            for (int j = g; j < g + kGroup; j += 4) {
+                float d[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const bool acc = d[u] < t[K - 1];
-                if (__ballot(acc)) {
-                    med3_insert<K>(t, d[u]);
+                for (int u = 0; u < 4; ++u) {
+                    const float4 c = stage[j + u];
+                    d[u] = norm2(qx, qy, qz, c.x, c.y, c.z);
+                }
+                // Sort the four ascending per lane (a five-exchange network on
+                // the bit patterns: non-negative floats order like unsigned
+                // integers), then insert them in order while some lane still
+                // accepts one.  Insertion order does not change the final
+                // set, and the chain runs as many times as the busiest lane
+                // needs rather than once per candidate that any lane accepts.
+                uint32_t a0 = __float_as_uint(d[0]), a1 = __float_as_uint(d[1]);
+                uint32_t a2 = __float_as_uint(d[2]), a3 = __float_as_uint(d[3]);
+                uint32_t lo = min(a0, a1), hi = max(a0, a1);
+                a0 = lo; a1 = hi;
+                lo = min(a2, a3); hi = max(a2, a3);
+                a2 = lo; a3 = hi;
+                lo = min(a0, a2); hi = max(a0, a2);
+                a0 = lo; a2 = hi;
+                lo = min(a1, a3); hi = max(a1, a3);
+                a1 = lo; a3 = hi;
+                lo = min(a1, a2); hi = max(a1, a2);
+                a1 = lo; a2 = hi;
+                const float sv[4] = {__uint_as_float(a0), __uint_as_float(a1), __uint_as_float(a2),
+                                     __uint_as_float(a3)};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool acc = sv[u] < t[K - 1];
+                    if (!__ballot(acc)) break;
+                    med3_insert<K>(t, sv[u]);
                     n_acc += acc ? 1u : 0u;
                 }
             }
@@ -331,9 +372,10 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
     uint32_t* __restrict__ wleaf_n, KnnStats* __restrict__ stats, const float* __restrict__ tau_init,
     const uint32_t* __restrict__ only_flagged, uint32_t* __restrict__ seed_redo,
-    unsigned long long* __restrict__ seed_fallbacks)
+    unsigned long long* __restrict__ seed_fallbacks, const Box* __restrict__ groups)
 {
     __shared__ float4 stage[kBoundWaves][kWave];
+    __shared__ Box gstage[kBoundWaves][kWave / kGroup];
     const int lane = threadIdx.x % kWave;
     const int wid = wave_in_block();
     const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBoundWaves + wid) * kWave;
@@ -374,8 +416,7 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     int stack = (lane == 0) ? 1 : 0;
     int sp = 1;
     for (;;) {
-        n_dist += static_cast<unsigned long long>(r_cnt);
-        bound_scan<K>(spos, r_start, r_cnt, stage[wid], lane, qx, qy, qz, t, n_acc);
+        bound_scan<K>(spos, r_start, r_cnt, stage[wid], groups, gstage[wid], lane, qx, qy, qz, t, n_acc, n_dist);
         bool found = false;
         while (sp > 0 && !found) {
             --sp;
@@ -818,17 +859,20 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
         return hipGetLastError();
     }
     if (leaf_size < 1 || leaf_size > kWave) return hipErrorInvalidValue;
+    // Group boxes line up with leaves (and with the 64-particle staging
+    // batches) only when the leaf size is a multiple of kGroup.
+    const Box* const groups = (leaf_size % kGroup == 0) ? scratch.groups : nullptr;
     const unsigned bound_blocks = static_cast<unsigned>((waves + kBoundWaves - 1) / kBoundWaves);
     mark_begin(0);
     knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n, stats,
-        scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks);
+        scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks, groups);
     ASM_HIP_CHECK(hipGetLastError());
     if (scratch.tau_init) {
         // Wavefronts with a lane whose learned bound was too small: again, unbounded.
         knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
             spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
-            stats, nullptr, scratch.seed_redo, nullptr, nullptr);
+            stats, nullptr, scratch.seed_redo, nullptr, nullptr, groups);
         ASM_HIP_CHECK(hipGetLastError());
     }
     mark_end(0);

@@ -87,6 +87,33 @@ __global__ void tree_leaves_kernel(const float4* __restrict__ spos, int64_t n, i
     nodes[nleaf_pad + j] = b;
 }
 
+// Box of each kGroup consecutive sorted particles: one thread per particle,
+// reduced over the kGroup lanes of a group with cross-lane exchanges.
+__global__ void tree_groups_kernel(const float4* __restrict__ spos, int64_t n, Box* __restrict__ groups)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t ngroups = (n + kGroup - 1) / kGroup;
+    if (i >= ngroups * kGroup) return;   // whole groups only: every lane of a group is present
+    const bool have = i < n;
+    const float4 q = spos[have ? i : n - 1];
+    float lo0 = have ? q.x : INFINITY, lo1 = have ? q.y : INFINITY, lo2 = have ? q.z : INFINITY;
+    float hi0 = have ? q.x : -INFINITY, hi1 = have ? q.y : -INFINITY, hi2 = have ? q.z : -INFINITY;
+#pragma unroll
+    for (int o = 1; o < kGroup; o <<= 1) {
+        lo0 = fminf(lo0, __shfl_xor(lo0, o)); hi0 = fmaxf(hi0, __shfl_xor(hi0, o));
+        lo1 = fminf(lo1, __shfl_xor(lo1, o)); hi1 = fmaxf(hi1, __shfl_xor(hi1, o));
+        lo2 = fminf(lo2, __shfl_xor(lo2, o)); hi2 = fmaxf(hi2, __shfl_xor(hi2, o));
+    }
+    if ((i & (kGroup - 1)) == 0) {
+        Box b;
+        b.lo[0] = lo0; b.lo[1] = lo1; b.lo[2] = lo2;
+        b.hi[0] = hi0; b.hi[1] = hi1; b.hi[2] = hi2;
+        b.lo_w = static_cast<int>(i);
+        b.hi_w = static_cast<int>((n - i < kGroup) ? n - i : kGroup);
+        groups[i / kGroup] = b;
+    }
+}
+
 __global__ void tree_level_kernel(Box* __restrict__ nodes, int64_t first, int64_t count)
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -127,9 +154,14 @@ hipError_t launch_gather_sorted(const ParticleRec* p, const uint32_t* perm, int6
 }
 
 hipError_t launch_build_tree(const float4* spos, int64_t n, int leaf_size, int64_t nleaf,
-                             int64_t nleaf_pad, Box* nodes, hipStream_t s)
+                             int64_t nleaf_pad, Box* nodes, Box* groups, hipStream_t s)
 {
     const int th = 256;
+    if (groups && n > 0) {
+        const int64_t m = (n + kGroup - 1) / kGroup * kGroup;
+        tree_groups_kernel<<<static_cast<unsigned>((m + th - 1) / th), th, 0, s>>>(spos, n, groups);
+        ASM_HIP_CHECK(hipGetLastError());
+    }
     tree_leaves_kernel<<<static_cast<unsigned>((nleaf_pad + th - 1) / th), th, 0, s>>>(
         spos, n, leaf_size, nleaf, nleaf_pad, nodes);
     ASM_HIP_CHECK(hipGetLastError());

@@ -64,7 +64,7 @@ def test_path_without_kept_lists_matches_oracle(ctx, kind, n_side, seed):
         _check_mesh(g["meshes"][nc], o["meshes"][nc], nc, vmax)
 
 
-@pytest.mark.parametrize("leaf_size", [8, 16, 64])
+@pytest.mark.parametrize("leaf_size", [8, 16, 20, 64])
 def test_leaf_size_does_not_change_results(ctx, leaf_size):
     xv, L, bf = small_set("blobs", 16, 7)
     parts, nl = A.build_buffered_set(xv, L, bf)
