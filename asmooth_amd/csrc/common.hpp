@@ -85,9 +85,13 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t w, uint32_t nblocks)
 }
 
 // Wavefront-wide helpers.
+__device__ __forceinline__ uint64_t wave_ballot(bool p)
+{
+    return __builtin_amdgcn_ballot_w64(p);
+}
 __device__ __forceinline__ bool wave_any(bool p)
 {
-    return __ballot(p) != 0ull;
+    return __builtin_amdgcn_ballot_w64(p) != 0ull;
 }
 __device__ __forceinline__ float wave_min(float v)
 {

@@ -275,42 +275,40 @@ __device__ __forceinline__ void med3_insert(float (&t)[K], float d)
 
 // Stage up to 64 particles of [first, first + left) in LDS; slots past the
 // end hold a point at infinity (squared distance +inf: never accepted), so
-// the scan loops can take whole groups of candidates.  With group boxes,
-// lanes 0..7 also stage the boxes of the 8 groups (first is a multiple of
-// kGroup); groups past the end get an empty box.  Returns the number of
-// slots to scan, rounded up to whole groups.
+// the scan loops can take whole groups of candidates.  With group boxes
+// (first is then a multiple of kGroup), the boxes of the 8 groups are staged
+// too: a Box is kGroup dwords, so group first / kGroup starts at dword
+// `first` and lane l loads dword first + l (clamped to the array; only groups
+// holding staged particles are ever read).  Returns the number of slots to
+// scan, rounded up to whole groups.
 __device__ __forceinline__ int stage_batch(const float4* __restrict__ spos, int64_t first, int64_t left,
-                                           float4* stage, const Box* __restrict__ groups, Box* gstage, int lane)
+                                           float4* stage, const uint32_t* __restrict__ gwords, int64_t gmax,
+                                           uint32_t* gstage, int lane)
 {
+    static_assert(sizeof(Box) == kGroup * sizeof(uint32_t), "a group box is kGroup dwords");
     const int m = static_cast<int>(left < kWave ? left : kWave);
-    stage[lane] = (lane < m) ? spos[first + lane] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
-    if (groups && lane < kWave / kGroup) {
-        Box b;
-        if (lane * kGroup < m) {
-            b = groups[first / kGroup + lane];
-        } else {
-            b.lo[0] = b.lo[1] = b.lo[2] = INFINITY;
-            b.hi[0] = b.hi[1] = b.hi[2] = -INFINITY;
-            b.lo_w = b.hi_w = 0;
-        }
-        gstage[lane] = b;
-    }
+    const float4 c = (lane < m) ? spos[first + lane] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    uint32_t gw = 0u;
+    if (gwords) gw = gwords[(first + lane < gmax) ? first + lane : gmax];
+    stage[lane] = c;
+    gstage[lane] = gw;
     wave_lds_fence();
     return (m + kGroup - 1) & ~(kGroup - 1);
 }
 
 template <int K>
 __device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int64_t start, int64_t cnt,
-                                           float4* stage, const Box* __restrict__ groups, Box* gstage, int lane,
-                                           float qx, float qy, float qz, float (&t)[K], uint32_t& n_acc,
-                                           unsigned long long& n_dist)
+                                           float4* stage, const uint32_t* __restrict__ gwords, int64_t gmax,
+                                           uint32_t* gstage, int lane, float qx, float qy, float qz, float (&t)[K],
+                                           uint32_t& n_acc, unsigned long long& n_dist)
 {
     for (int64_t base = 0; base < cnt; base += kWave) {
-        const int m8 = stage_batch(spos, start + base, cnt - base, stage, groups, gstage, lane);
+        const int m8 = stage_batch(spos, start + base, cnt - base, stage, gwords, gmax, gstage, lane);
         for (int g = 0; g < m8; g += kGroup) {
             // Skip a group whose box no lane's current bound reaches: every
             // candidate in it is at least the box distance away.
-            if (groups && !wave_any(box_d2(gstage[g / kGroup], qx, qy, qz) <= t[K - 1])) continue;
+            if (gwords && !wave_any(box_d2(*reinterpret_cast<const Box*>(gstage + g), qx, qy, qz) <= t[K - 1]))
+                continue;
             n_dist += kGroup;
 #pragma unroll
             for (int j = g; j < g + kGroup; j += 4) {
@@ -343,7 +341,7 @@ __device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int6
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const bool acc = sv[u] < t[K - 1];
-                    if (!__ballot(acc)) break;
+                    if (!wave_any(acc)) break;
                     med3_insert<K>(t, sv[u]);
                     n_acc += acc ? 1u : 0u;
                 }
@@ -367,7 +365,7 @@ __device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int6
 // result is exact; otherwise the lane's wavefront is flagged in seed_redo
 // and searched again with only_flagged = seed_redo and no bound.
 template <int K>
-__global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
+__global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
     int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
     uint32_t* __restrict__ wleaf_n, KnnStats* __restrict__ stats, const float* __restrict__ tau_init,
@@ -375,7 +373,7 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     unsigned long long* __restrict__ seed_fallbacks, const Box* __restrict__ groups)
 {
     __shared__ float4 stage[kBoundWaves][kWave];
-    __shared__ Box gstage[kBoundWaves][kWave / kGroup];
+    __shared__ __align__(16) uint32_t gstage[kBoundWaves][kWave];
     const int lane = threadIdx.x % kWave;
     const int wid = wave_in_block();
     const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kBoundWaves + wid) * kWave;
@@ -386,15 +384,20 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     const bool valid = q < n;
     const float4 me = spos[valid ? q : n - 1];
     const float qx = me.x, qy = me.y, qz = me.z;
-    const float t0 = (tau_init && valid) ? tau_init[q] : INFINITY;
+    // Lanes past the end repeat the last particle (position and bound), so
+    // they never need a node or a group that its lane does not need: the
+    // wavefront-wide tests below need no validity mask.
+    const float t0 = tau_init ? tau_init[valid ? q : n - 1] : INFINITY;
 
     float t[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) t[i] = t0;
+    const uint32_t* const gwords = reinterpret_cast<const uint32_t*>(groups);
+    const int64_t gmax = (n + kGroup - 1) / kGroup * kGroup - 1;   // last dword of the group boxes
 
     unsigned long long n_node = 0, n_leaf = 0, n_dist = 0;
     uint32_t n_acc = 0;
-    const unsigned long long n_valid_lanes = __popcll(__ballot(valid));
+    const unsigned long long n_valid_lanes = __popcll(wave_ballot(valid));
 
     const int64_t last = (n < wave0 + kWave ? n : wave0 + kWave) - 1;
     int64_t l0 = wave0 / leaf_size - init_leaves;
@@ -416,7 +419,8 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     int stack = (lane == 0) ? 1 : 0;
     int sp = 1;
     for (;;) {
-        bound_scan<K>(spos, r_start, r_cnt, stage[wid], groups, gstage[wid], lane, qx, qy, qz, t, n_acc, n_dist);
+        bound_scan<K>(spos, r_start, r_cnt, stage[wid], gwords, gmax, gstage[wid], lane, qx, qy, qz, t, n_acc,
+                      n_dist);
         bool found = false;
         while (sp > 0 && !found) {
             --sp;
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
             const Box b = nodes[node];
             ++n_node;
             const float kth = t[K - 1];
-            if (!wave_any(valid && box_d2(b, qx, qy, qz) <= kth)) continue;
+            if (!wave_any(box_d2(b, qx, qy, qz) <= kth)) continue;
             if (node >= nleaf_pad) {
                 const int64_t leaf = node - nleaf_pad;
                 if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
@@ -442,10 +446,10 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
                 const Box c1 = nodes[2 * node + 1];
                 const float d0 = box_d2(c0, qx, qy, qz);
                 const float d1 = box_d2(c1, qx, qy, qz);
-                const bool need0 = wave_any(valid && d0 <= kth);
-                const bool need1 = wave_any(valid && d1 <= kth);
-                const int votes0 = __popcll(__ballot(valid && d0 <= d1));
-                const bool near0 = 2 * votes0 >= static_cast<int>(n_valid_lanes);
+                const bool need0 = wave_any(d0 <= kth);
+                const bool need1 = wave_any(d1 <= kth);
+                const int votes0 = __popcll(wave_ballot(d0 <= d1));
+                const bool near0 = 2 * votes0 >= kWave;
                 const int first = near0 ? 2 * node + 1 : 2 * node;   // pushed first = visited last
                 const int second = near0 ? 2 * node : 2 * node + 1;
                 const bool need_first = near0 ? need1 : need0;
@@ -467,7 +471,7 @@ __global__ __launch_bounds__(kBoundThreads) void knn_bound_kernel(
     if (lane == 0) wleaf_n[wave] = static_cast<uint32_t>(n_rec);
     if (valid) tau_out[q] = t[K - 1];
     if (tau_init) {
-        const uint64_t fb = __ballot(valid && !(t[K - 1] < t0));
+        const uint64_t fb = wave_ballot(valid && !(t[K - 1] < t0));
         if (lane == 0) {
             seed_redo[wave] = fb ? 1u : 0u;
             if (fb) atomicAdd(&seed_fallbacks[wave & (kStatSlots - 1)], static_cast<unsigned long long>(__popcll(fb)));
