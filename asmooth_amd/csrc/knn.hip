@@ -281,12 +281,12 @@ __device__ __forceinline__ void med3_insert(float (&t)[K], float d)
 // `first` and lane l loads dword first + l (clamped to the array; only groups
 // holding staged particles are ever read).  Returns the number of slots to
 // scan, rounded up to whole groups.
-__device__ __forceinline__ int stage_batch(const float4* __restrict__ spos, int64_t first, int64_t left,
-                                           float4* stage, const uint32_t* __restrict__ gwords, int64_t gmax,
+__device__ __forceinline__ int stage_batch(const float4* __restrict__ spos, int first, int left,
+                                           float4* stage, const uint32_t* __restrict__ gwords, int gmax,
                                            uint32_t* gstage, int lane)
 {
     static_assert(sizeof(Box) == kGroup * sizeof(uint32_t), "a group box is kGroup dwords");
-    const int m = static_cast<int>(left < kWave ? left : kWave);
+    const int m = left < kWave ? left : kWave;
     const float4 c = (lane < m) ? spos[first + lane] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
     uint32_t gw = 0u;
     if (gwords) gw = gwords[(first + lane < gmax) ? first + lane : gmax];
@@ -297,12 +297,12 @@ __device__ __forceinline__ int stage_batch(const float4* __restrict__ spos, int6
 }
 
 template <int K>
-__device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int64_t start, int64_t cnt,
-                                           float4* stage, const uint32_t* __restrict__ gwords, int64_t gmax,
+__device__ __forceinline__ void bound_scan(const float4* __restrict__ spos, int start, int cnt,
+                                           float4* stage, const uint32_t* __restrict__ gwords, int gmax,
                                            uint32_t* gstage, int lane, float qx, float qy, float qz, float (&t)[K],
-                                           uint32_t& n_acc, unsigned long long& n_dist)
+                                           uint32_t& n_acc, uint32_t& n_dist)
 {
-    for (int64_t base = 0; base < cnt; base += kWave) {
+    for (int base = 0; base < cnt; base += kWave) {
         const int m8 = stage_batch(spos, start + base, cnt - base, stage, gwords, gmax, gstage, lane);
         for (int g = 0; g < m8; g += kGroup) {
             // Skip a group whose box no lane's current bound reaches: every
@@ -392,18 +392,24 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
     float t[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) t[i] = t0;
-    const uint32_t* const gwords = reinterpret_cast<const uint32_t*>(groups);
-    const int64_t gmax = (n + kGroup - 1) / kGroup * kGroup - 1;   // last dword of the group boxes
 
-    unsigned long long n_node = 0, n_leaf = 0, n_dist = 0;
-    uint32_t n_acc = 0;
+    // Particle, node and leaf indices fit in 32 bits (the launcher checks
+    // n); the walk keeps them in 32-bit scalars.
+    const int ni = static_cast<int>(n);
+    const int w0 = static_cast<int>(wave0);
+    const int nlp = static_cast<int>(nleaf_pad);
+    const uint32_t* const gwords = reinterpret_cast<const uint32_t*>(groups);
+    const int gmax = (ni + kGroup - 1) / kGroup * kGroup - 1;   // last dword of the group boxes
+
+    uint32_t n_node = 0, n_dist = 0, n_acc = 0;
     const unsigned long long n_valid_lanes = __popcll(wave_ballot(valid));
 
-    const int64_t last = (n < wave0 + kWave ? n : wave0 + kWave) - 1;
-    int64_t l0 = wave0 / leaf_size - init_leaves;
-    int64_t l1 = last / leaf_size + init_leaves;
+    const int last = (ni < w0 + kWave ? ni : w0 + kWave) - 1;
+    int l0 = w0 / leaf_size - init_leaves;
+    int l1 = last / leaf_size + init_leaves;
     if (l0 < 0) l0 = 0;
-    if (l1 > nleaf - 1) l1 = nleaf - 1;
+    if (l1 > static_cast<int>(nleaf) - 1) l1 = static_cast<int>(nleaf) - 1;
+    uint32_t n_leaf = static_cast<uint32_t>(l1 - l0 + 1);
 
     // Recorded leaf ids: 64 at a time in one VGPR (lane i holds entry i),
     // flushed with one coalesced store.
@@ -413,9 +419,11 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
 
     // One call site of the scan (the seed range first, then each leaf the
     // walk opens), so the compiler keeps a single copy of t.
-    int64_t r_start = l0 * leaf_size;
-    int64_t r_cnt = (((l1 + 1) * leaf_size < n) ? (l1 + 1) * leaf_size : n) - r_start;
-    n_leaf += static_cast<unsigned long long>(l1 - l0 + 1);
+    int r_start = l0 * leaf_size;
+    int r_cnt = (((l1 + 1) * leaf_size < ni) ? (l1 + 1) * leaf_size : ni) - r_start;
+    // Traversal stack in one VGPR: lane i holds entry i (a depth-first walk
+    // of the binary tree keeps at most one entry per level plus one, well
+    // below 64).
     int stack = (lane == 0) ? 1 : 0;
     int sp = 1;
     for (;;) {
@@ -429,8 +437,8 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
             ++n_node;
             const float kth = t[K - 1];
             if (!wave_any(box_d2(b, qx, qy, qz) <= kth)) continue;
-            if (node >= nleaf_pad) {
-                const int64_t leaf = node - nleaf_pad;
+            if (node >= nlp) {
+                const int leaf = node - nlp;
                 if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
                 ++n_leaf;
                 r_start = b.lo_w;
@@ -442,26 +450,24 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
                 }
                 ++n_rec;
             } else {
-                const Box c0 = nodes[2 * node];
-                const Box c1 = nodes[2 * node + 1];
-                const float d0 = box_d2(c0, qx, qy, qz);
-                const float d1 = box_d2(c1, qx, qy, qz);
+                const int c0 = 2 * node;
+                const Box b0 = nodes[c0];
+                const Box b1 = nodes[c0 + 1];
+                const float d0 = box_d2(b0, qx, qy, qz);
+                const float d1 = box_d2(b1, qx, qy, qz);
                 const bool need0 = wave_any(d0 <= kth);
                 const bool need1 = wave_any(d1 <= kth);
-                const int votes0 = __popcll(wave_ballot(d0 <= d1));
-                const bool near0 = 2 * votes0 >= kWave;
-                const int first = near0 ? 2 * node + 1 : 2 * node;   // pushed first = visited last
-                const int second = near0 ? 2 * node : 2 * node + 1;
-                const bool need_first = near0 ? need1 : need0;
-                const bool need_second = near0 ? need0 : need1;
-                if (need_first) {
-                    stack = (lane == sp) ? first : stack;
-                    ++sp;
-                }
-                if (need_second) {
-                    stack = (lane == sp) ? second : stack;
-                    ++sp;
-                }
+                // Near child (by lane vote) on top, visited first.  Slot sp
+                // takes the far child when it is needed (else the near one),
+                // slot sp + 1 the near one; sp advances by the number needed,
+                // so an unused slot lies above the stack top.
+                const bool near0 = 2 * __popcll(wave_ballot(d0 <= d1)) >= kWave;
+                const int nearc = near0 ? c0 : c0 + 1;
+                const bool need_near = near0 ? need0 : need1;
+                const bool need_far = near0 ? need1 : need0;
+                stack = (lane == sp) ? (need_far ? (nearc ^ 1) : nearc) : stack;
+                stack = (lane == sp + 1) ? nearc : stack;
+                sp += static_cast<int>(need_far) + static_cast<int>(need_near);
             }
         }
         if (!found) break;
@@ -483,9 +489,9 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
         KnnStats* const st = stats + (wave & (kStatSlots - 1));
         if (lane == 0) {
-            atomicAdd(&st->node_tests, n_node);
-            atomicAdd(&st->leaf_scans, n_leaf);
-            atomicAdd(&st->dist_evals, n_dist * n_valid_lanes);
+            atomicAdd(&st->node_tests, static_cast<unsigned long long>(n_node));
+            atomicAdd(&st->leaf_scans, static_cast<unsigned long long>(n_leaf));
+            atomicAdd(&st->dist_evals, static_cast<unsigned long long>(n_dist) * n_valid_lanes);
             atomicAdd(&st->inserts, v);
         }
     }
@@ -863,6 +869,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
         return hipGetLastError();
     }
     if (leaf_size < 1 || leaf_size > kWave) return hipErrorInvalidValue;
+    if (n > 0x7fffffffLL - 2 * kWave) return hipErrorInvalidValue;   // 32-bit indices in the walks
     // Group boxes line up with leaves (and with the 64-particle staging
     // batches) only when the leaf size is a multiple of kGroup.
     const Box* const groups = (leaf_size % kGroup == 0) ? scratch.groups : nullptr;
