@@ -426,37 +426,61 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
     // below 64).
     int stack = (lane == 0) ? 1 : 0;
     int sp = 1;
+    const auto record = [&](int leaf) {
+        if (n_rec < kLeafCap) {
+            rec = (lane == (n_rec & (kWave - 1))) ? static_cast<uint32_t>(leaf) : rec;
+            if ((n_rec & (kWave - 1)) == kWave - 1) my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
+        }
+        ++n_rec;
+    };
     for (;;) {
         bound_scan<K>(spos, r_start, r_cnt, stage[wid], gwords, gmax, gstage[wid], lane, qx, qy, qz, t, n_acc,
                       n_dist);
+        // The stack top was tested against the current bound when it was
+        // pushed, until the next scan lowers the bound.
+        bool fresh = false;
         bool found = false;
         while (sp > 0 && !found) {
             --sp;
             const int node = __builtin_amdgcn_readlane(stack, sp);
-            const Box b = nodes[node];
             ++n_node;
             const float kth = t[K - 1];
-            if (!wave_any(box_d2(b, qx, qy, qz) <= kth)) continue;
             if (node >= nlp) {
+                // A leaf is only ever popped as the root of a one-leaf tree.
+                const Box b = nodes[node];
+                if (!wave_any(box_d2(b, qx, qy, qz) <= kth)) continue;
                 const int leaf = node - nlp;
                 if (leaf >= l0 && leaf <= l1) continue;  // scanned while seeding
                 ++n_leaf;
                 r_start = b.lo_w;
                 r_cnt = b.hi_w;
                 found = true;
-                if (n_rec < kLeafCap) {
-                    rec = (lane == (n_rec & (kWave - 1))) ? static_cast<uint32_t>(leaf) : rec;
-                    if ((n_rec & (kWave - 1)) == kWave - 1) my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
-                }
-                ++n_rec;
+                record(leaf);
+                continue;
+            }
+            if (!fresh && !wave_any(box_d2(nodes[node], qx, qy, qz) <= kth)) continue;
+            fresh = false;
+            const int c0 = 2 * node;
+            const Box b0 = nodes[c0];
+            const Box b1 = nodes[c0 + 1];
+            const float d0 = box_d2(b0, qx, qy, qz);
+            const float d1 = box_d2(b1, qx, qy, qz);
+            const bool need0 = wave_any(d0 <= kth);
+            const bool need1 = wave_any(d1 <= kth);
+            if (c0 >= nlp) {
+                // Both children are leaves, adjacent in particle order: the
+                // needed ones (outside the seed range) are scanned together.
+                const int la = c0 - nlp;
+                const bool sa = need0 && !(la >= l0 && la <= l1);
+                const bool sb = need1 && !(la + 1 >= l0 && la + 1 <= l1);
+                if (!sa && !sb) continue;
+                r_start = sa ? b0.lo_w : b1.lo_w;
+                r_cnt = (sa ? b0.hi_w : 0) + (sb ? b1.hi_w : 0);
+                n_leaf += static_cast<uint32_t>(sa) + static_cast<uint32_t>(sb);
+                if (sa) record(la);
+                if (sb) record(la + 1);
+                found = true;
             } else {
-                const int c0 = 2 * node;
-                const Box b0 = nodes[c0];
-                const Box b1 = nodes[c0 + 1];
-                const float d0 = box_d2(b0, qx, qy, qz);
-                const float d1 = box_d2(b1, qx, qy, qz);
-                const bool need0 = wave_any(d0 <= kth);
-                const bool need1 = wave_any(d1 <= kth);
                 // Near child (by lane vote) on top, visited first.  Slot sp
                 // takes the far child when it is needed (else the near one),
                 // slot sp + 1 the near one; sp advances by the number needed,
@@ -468,6 +492,7 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
                 stack = (lane == sp) ? (need_far ? (nearc ^ 1) : nearc) : stack;
                 stack = (lane == sp + 1) ? nearc : stack;
                 sp += static_cast<int>(need_far) + static_cast<int>(need_near);
+                fresh = need_far || need_near;
             }
         }
         if (!found) break;
