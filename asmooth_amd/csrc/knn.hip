@@ -287,9 +287,16 @@ __device__ __forceinline__ int stage_batch(const float4* __restrict__ spos, int 
 {
     static_assert(sizeof(Box) == kGroup * sizeof(uint32_t), "a group box is kGroup dwords");
     const int m = left < kWave ? left : kWave;
-    const float4 c = (lane < m) ? spos[first + lane] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    // Addresses as a uniform base plus a 32-bit lane offset (no 64-bit
+    // per-lane pointers held across the walk).
+    const uint32_t idx = static_cast<uint32_t>(first + lane);
+    const float4 c = (lane < m) ? *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(spos) + idx * 16u)
+                                : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
     uint32_t gw = 0u;
-    if (gwords) gw = gwords[(first + lane < gmax) ? first + lane : gmax];
+    if (gwords) {
+        const uint32_t gi = idx < static_cast<uint32_t>(gmax) ? idx : static_cast<uint32_t>(gmax);
+        gw = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(gwords) + gi * 4u);
+    }
     stage[lane] = c;
     gstage[lane] = gw;
     wave_lds_fence();
@@ -460,6 +467,44 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
             }
             if (!fresh && !wave_any(box_d2(nodes[node], qx, qy, qz) <= kth)) continue;
             fresh = false;
+            if (4 * node >= nlp && 2 * node < nlp) {
+                // The grandchildren are four leaves, consecutive in particle
+                // order: test their boxes directly and scan the span from the
+                // first needed one to the last (the group boxes weed out the
+                // rest of the span); record only the needed ones.  A span
+                // holding a leaf of the seed range (already scanned) takes
+                // the two-level path below instead.
+                const int g0 = 4 * node;
+                const int la = g0 - nlp;
+                uint32_t mask = 0, seeded = 0;
+                Box lb[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    lb[j] = nodes[g0 + j];
+                    const bool sj = la + j >= l0 && la + j <= l1;
+                    seeded |= sj ? (1u << j) : 0u;
+                    if (!sj && wave_any(box_d2(lb[j], qx, qy, qz) <= kth)) mask |= 1u << j;
+                }
+                if (mask == 0u) continue;
+                const int jlo = __builtin_ctz(mask), jhi = 31 - __builtin_clz(mask);
+                const uint32_t span = (2u << jhi) - (1u << jlo);
+                if ((span & seeded) == 0u) {
+                    int lo_w = lb[0].lo_w, hi_end = lb[0].lo_w + lb[0].hi_w;
+#pragma unroll
+                    for (int j = 1; j < 4; ++j) {
+                        lo_w = (j == jlo) ? lb[j].lo_w : lo_w;
+                        hi_end = (j == jhi) ? lb[j].lo_w + lb[j].hi_w : hi_end;
+                    }
+                    r_start = lo_w;
+                    r_cnt = hi_end - lo_w;
+                    n_leaf += static_cast<uint32_t>(__popc(mask));
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (mask & (1u << j)) record(la + j);
+                    found = true;
+                    continue;
+                }
+            }
             const int c0 = 2 * node;
             const Box b0 = nodes[c0];
             const Box b1 = nodes[c0 + 1];
