@@ -477,11 +477,16 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
 // entry cover consecutive x cells (consecutive LDS words), every lane does
 // useful work whatever the box shapes, and each term is the reference's
 // FP32 product ((wx * wy) * wz, coarse_mesh.cpp:93).
-constexpr int kDepWaves = kDepThreads / kWave;
+//
+// Eight wavefronts share a tile's accumulators: the tile's FP64 sums (24 KB)
+// plus 6.5 KB of weights and marks per wavefront let two workgroups, 16
+// wavefronts, share a CU's LDS.
+constexpr int kPackThreads = 512;
+constexpr int kDepWaves = kPackThreads / kWave;
 constexpr int kWStride = 3 * kTileMax + 1;   // odd row stride: entries fall in different banks
 
 template <typename Acc>
-__global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
+__global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
     DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
     const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
 {
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_packed_kernel(
     const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
     const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
     const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
-    for (int c = tid; c < 6 * kTileCells; c += kDepThreads) (&acc[0][0])[c] = Acc(0);
+    for (int c = tid; c < 6 * kTileCells; c += kPackThreads) (&acc[0][0])[c] = Acc(0);
     map[lane] = -1;
     __syncthreads();
 
@@ -814,10 +819,10 @@ hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start
         deposit_tile_kahan_kernel<<<static_cast<unsigned>(ntiles), kKahanThreads, 0, s>>>(a, tile_start, tile_cnt,
                                                                                           list, mesh, accumulate);
     else if (a.accum == kAccumAtomic32)
-        deposit_tile_packed_kernel<float><<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(
+        deposit_tile_packed_kernel<float><<<static_cast<unsigned>(ntiles), kPackThreads, 0, s>>>(
             a, tile_start, tile_cnt, list, mesh, accumulate);
     else
-        deposit_tile_packed_kernel<double><<<static_cast<unsigned>(ntiles), kDepThreads, 0, s>>>(
+        deposit_tile_packed_kernel<double><<<static_cast<unsigned>(ntiles), kPackThreads, 0, s>>>(
             a, tile_start, tile_cnt, list, mesh, accumulate);
     return hipGetLastError();
 }
