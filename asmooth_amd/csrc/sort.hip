@@ -74,6 +74,8 @@ __global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint32_t* __res
     __shared__ uint32_t wsum[kSortThreads / kWave];
     __shared__ uint32_t gbase[kRadix];
     __shared__ int64_t gofs[kRadix];
+    __shared__ uint32_t sk[kSortTile];   // the tile in digit order, staged for coalesced stores
+    __shared__ uint32_t sv[kSortTile];
     const int t = threadIdx.x;
 #pragma unroll
     for (int d = 0; d < kRadix; ++d) cnt[d * kCntStride + t] = 0u;
@@ -135,14 +137,26 @@ __global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint32_t* __res
     // taken before any thread advances its own counter.
     if (t < kRadix) gofs[t] = static_cast<int64_t>(gbase[t]) - static_cast<int64_t>(cnt[t * kCntStride]);
     __syncthreads();
+    // Place every key at its in-tile rank in LDS, then store the tile in
+    // rank order: consecutive ranks of one digit go to consecutive
+    // addresses, so the stores coalesce.
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         if (j >= nvalid) break;
         const uint32_t d = (k[j] >> shift) & (kRadix - 1);
-        const int64_t pos = gofs[d] + cnt[d * kCntStride + t];
-        cnt[d * kCntStride + t] += 1u;
-        kout[pos] = k[j];
-        vout[pos] = v[j];
+        const uint32_t r = cnt[d * kCntStride + t];
+        cnt[d * kCntStride + t] = r + 1u;
+        sk[r] = k[j];
+        sv[r] = v[j];
+    }
+    __syncthreads();
+    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    const int tile_n = static_cast<int>(n - tile0 < kSortTile ? n - tile0 : kSortTile);
+    for (int r = t; r < tile_n; r += kSortThreads) {
+        const uint32_t key = sk[r];
+        const int64_t pos = gofs[(key >> shift) & (kRadix - 1)] + r;
+        kout[pos] = key;
+        vout[pos] = sv[r];
     }
 }
 
