@@ -211,6 +211,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--seed-margin", type=float, default=0.3)
+    ap.add_argument("--batch-streams", type=int, default=8, help="streams the batched graph forks into")
     ap.add_argument("--leaf-size", type=int, default=32, help="particles per tree leaf (8..64)")
     ap.add_argument("--tile", type=int, default=8, help="deposition tile side in cells (1..8)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -300,7 +301,7 @@ def main(argv=None) -> int:
     ctx.close()
     batched = None
     if not args.no_batched:
-        batched = batched_leg(max(2, min(args.steps, 5)))
+        batched = batched_leg(max(2, min(args.steps, 5)), args.batch_streams)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(xv, L, bf, ncs)

@@ -107,6 +107,26 @@ def test_minimum_size_and_duplicates(ctx):
     assert rel_err(dens, O.density(pos, onbr, ork)) <= DENS_RTOL
 
 
+@pytest.mark.parametrize("n,leaf", [(33, 32), (33, 64), (64, 32), (65, 32), (100, 32), (128, 32), (129, 32),
+                                    (200, 8), (300, 16), (520, 64)])
+def test_small_trees_match_oracle(ctx, n, leaf):
+    # Trees of one, two, four and more leaves: a one-leaf root, a root whose
+    # children are leaves, four-leaf spans at the root, spans that touch the
+    # seed range, and leaf sizes with and without group boxes.
+    rng = np.random.default_rng(n * 100 + leaf)
+    pos = rng.uniform(0, 10, (n, 3)).astype(np.float32)
+    pos[n // 2:] *= np.float32(0.3)   # denser half: uneven leaves
+    ctx.set_option(A.OPT_LEAF_SIZE, leaf)
+    try:
+        rk, dens, nbr = _knn_run(ctx, pos, 0)
+    finally:
+        ctx.set_option(A.OPT_LEAF_SIZE, 32)
+    onbr, _, ork = O.knn(pos)
+    assert np.array_equal(rk, ork)
+    assert np.array_equal(nbr, onbr)
+    assert rel_err(dens, O.density(pos, onbr, ork)) <= DENS_RTOL
+
+
 def _knn_run(ctx, pos, method, deterministic=0, keep=1):
     parts = A.particles_from_xv(np.concatenate([pos, np.zeros_like(pos)], 1))
     ctx.set_option(A.OPT_KEEP_NEIGHBORS, keep)
