@@ -1180,10 +1180,24 @@ int asm_batch_capture(asm_batch* b, int n_streams)
     for (auto& c : b->cubes) HIPC2(b, hipStreamSynchronize(c.ctx->stream));
     // Capture: fork from main to the share streams, join back.
     const int ns = std::min<int>(n_streams, static_cast<int>(b->cubes.size()));
-    std::vector<hipEvent_t> ev(ns + 1);
-    for (auto& e : ev) HIPC2(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    std::vector<hipEvent_t> ev(ns + 1, nullptr);
+    const auto destroy_events = [&ev]() {
+        for (auto& x : ev)
+            if (x) (void)hipEventDestroy(x);
+    };
+    for (auto& x : ev) {
+        const hipError_t ce = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+        if (ce != hipSuccess) {
+            destroy_events();
+            return batch_fail(b, ASM_ERR_HIP, "%s", hipGetErrorString(ce));
+        }
+    }
     int rc = ASM_OK;
-    HIPC2(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeRelaxed));
+    const hipError_t cb = hipStreamBeginCapture(b->main, hipStreamCaptureModeRelaxed);
+    if (cb != hipSuccess) {
+        destroy_events();
+        return batch_fail(b, ASM_ERR_HIP, "%s", hipGetErrorString(cb));
+    }
     hipError_t e = hipEventRecord(ev[ns], b->main);
     // Shares on stream s = i % ns run in order on that stream.
     std::vector<hipStream_t> lane_stream(ns);
@@ -1207,7 +1221,7 @@ int asm_batch_capture(asm_batch* b, int n_streams)
     }
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(b->main, &g);
-    for (auto& x : ev) (void)hipEventDestroy(x);
+    destroy_events();
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;

@@ -8,6 +8,10 @@ copied byte gives that kernel's HBM bytes per launch.
 
     rocprofv3 --pmc FETCH_SIZE --kernel-trace -d <dir> -o fetch -- \\
         python3 -m asmooth_amd.profile_traffic
+
+`summarize` turns a kernel-trace stats file and the two counter files into
+the per-kernel traffic table; `sq` turns an SQ counter file into the
+per-wavefront instruction table.
 """
 from __future__ import annotations
 
@@ -74,7 +78,43 @@ def summarize(stats_csv: str, fetch_csv: str, write_csv: str, commands: str, jso
     return "\n".join(out) + "\n"
 
 
+def summarize_sq(counters_csv: str, command: str) -> str:
+    """Markdown table of SQ instruction counters per wavefront (counter /
+    SQ_WAVES over all dispatches of a kernel), the fraction of wave cycles
+    spent waiting on instruction dependencies, and LDS bank-conflict cycles
+    per wavefront."""
+    import csv
+    import collections
+
+    tot = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    for r in csv.DictReader(open(counters_csv)):
+        k = _short(r["Kernel_Name"])
+        tot[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp[k].add(r["Dispatch_Id"])
+    out = ["Command (from the repository root, TMPDIR=/tmp):", "", f"    {command}", "",
+           "Per-wavefront instruction counts (counter / SQ_WAVES over all dispatches); 'wait' = "
+           "SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES; LDS bank-conflict cycles per wavefront.", "",
+           "| kernel | dispatches | wavefronts/dispatch | VALU/wave | SALU/wave | LDS/wave | wait | "
+           "LDS conflict cycles/wave |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    rows = sorted(tot.items(), key=lambda kv: -kv[1].get("SQ_WAVE_CYCLES", 0.0))
+    for k, c in rows:
+        w = c.get("SQ_WAVES", 0.0)
+        if w <= 0:
+            continue
+        nd = len(disp[k])
+        wait = c.get("SQ_WAIT_INST_ANY", 0.0) / c["SQ_WAVE_CYCLES"] if c.get("SQ_WAVE_CYCLES") else 0.0
+        out.append(f"| {k} | {nd} | {w / nd:.0f} | {c.get('SQ_INSTS_VALU', 0.0) / w:.0f} | "
+                   f"{c.get('SQ_INSTS_SALU', 0.0) / w:.0f} | {c.get('SQ_INSTS_LDS', 0.0) / w:.0f} | "
+                   f"{100.0 * wait:.0f}% | {c.get('SQ_LDS_BANK_CONFLICT', 0.0) / w:.0f} |")
+    return "\n".join(out) + "\n"
+
+
 def main() -> int:
+    if len(sys.argv) > 1 and sys.argv[1] == "sq":
+        # sq <counters.csv> "<command>"
+        print(summarize_sq(sys.argv[2], sys.argv[3]))
+        return 0
     if len(sys.argv) > 1 and sys.argv[1] == "summarize":
         # summarize <stats.csv> <fetch.csv> <write.csv> "<cmd1> ;; <cmd2> ;; ..." [json_out]
         print(summarize(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5],
