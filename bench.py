@@ -54,11 +54,16 @@ def measured_traffic(kernel: str):
     workload, or None."""
     try:
         with open(TRAFFIC_FILE) as f:
-            k = json.load(f)["kernels"].get(kernel)
+            kernels = json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
-    if not k or k.get("read_bytes") is None or k.get("write_bytes") is None:
+    # Entries are named with their template arguments ("knn_bound_kernel<33>");
+    # of several instantiations, take the one launched most often.
+    cands = [v for name, v in kernels.items() if name.split("<")[0] == kernel
+             and v.get("read_bytes") is not None and v.get("write_bytes") is not None]
+    if not cands:
         return None
+    k = max(cands, key=lambda v: v.get("calls", 0))
     return int(k["read_bytes"] + k["write_bytes"])
 
 
