@@ -107,7 +107,6 @@ struct asm_context {
     int deposit_accum = kAccumFp64;
     DevBuf<uint32_t> knn_flags, knn_leaf_n, knn_leaves;   // per wavefront of 64 sorted particles
     DevBuf<float> knn_tau;                                // per sorted particle
-    DevBuf<float> knn_hinv;                               // per sorted particle: 1/h for the collect walk
     DevBuf<double> dens_pull;
     // learned seed radius
     bool seed_on = false;
@@ -436,10 +435,8 @@ int asm_neighbor_search(asm_context* ctx, int k)
         HIPC(ctx->knn_tau.ensure(n));
         if (!ctx->deterministic) HIPC(ctx->dens_pull.ensure(n));
     }
-    if (bound_collect) HIPC(ctx->knn_hinv.ensure(n));
     KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p,
-                    nullptr, nullptr, nullptr, ctx->leaf_size % kGroup == 0 ? ctx->groups.p : nullptr,
-                    ctx->knn_hinv.p};
+                    nullptr, nullptr, nullptr, ctx->leaf_size % kGroup == 0 ? ctx->groups.p : nullptr};
     ctx->seed_used = false;
     if (bound_collect && ctx->seed_on) {
         HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));

@@ -78,9 +78,6 @@ struct KnnScratch {
     // by the bound walk to skip groups of an opened leaf; null when the leaf
     // size is not a multiple of kGroup.
     const Box* groups;
-    // Per sorted particle 1/h = 1/sqrtf(tau), written by the bound walk and
-    // read by the collect walk for the terms of the particles it visits.
-    float* hinv;
 };
 // Optional marks around the kernels of the bound/collect method (stage 0:
 // bound walk, 1: collect walk, 2: insertion kernel for flagged wavefronts).

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
     int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
     uint32_t* __restrict__ wleaf_n, KnnStats* __restrict__ stats, const float* __restrict__ tau_init,
     const uint32_t* __restrict__ only_flagged, uint32_t* __restrict__ seed_redo,
-    unsigned long long* __restrict__ seed_fallbacks, const Box* __restrict__ groups, float* __restrict__ hinv_out)
+    unsigned long long* __restrict__ seed_fallbacks, const Box* __restrict__ groups)
 {
     __shared__ float4 stage[kBoundWaves][kWave];
     __shared__ __align__(16) uint32_t gstage[kBoundWaves][kWave];
@@ -548,10 +548,7 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
     if (n_rec <= kLeafCap && (n_rec & (kWave - 1)) != 0 && lane < (n_rec & (kWave - 1)))
         my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
     if (lane == 0) wleaf_n[wave] = static_cast<uint32_t>(n_rec);
-    if (valid) {
-        tau_out[q] = t[K - 1];
-        hinv_out[q] = 1.0f / sqrtf(t[K - 1]);   // 1/h exactly as the particle's own collect lane computes it
-    }
+    if (valid) tau_out[q] = t[K - 1];
     if (tau_init) {
         const uint64_t fb = wave_ballot(valid && !(t[K - 1] < t0));
         if (lane == 0) {
@@ -600,8 +597,8 @@ struct CollectLds {
 //   * terms at exactly tau_o depend on index order and are left to the end.
 template <int M, int P, bool kAll>
 __device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int count, uint64_t mask, float4 c,
-                                             float tc, float hc, int lane, float qx, float qy, float qz, float tau,
-                                             float hinv, double* __restrict__ dens64, unsigned long long& n_pairs)
+                                             float tc, int lane, float qx, float qy, float qz, float tau, float hinv,
+                                             double* __restrict__ dens64, unsigned long long& n_pairs)
 {
     constexpr int G = kWave / P;
     const int j = lane % P;
@@ -616,6 +613,7 @@ __device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int co
         L.otab[k] = static_cast<uint32_t>(lane);
     }
     wave_lds_fence();
+    const float hc = 1.0f / sqrtf(tc);   // c's 1/h, as c's own lane computes it
     for (int k = grp; k < a; k += G) {
         const float4 o = L.qtab[k];
         const float d2 = norm2(o.x, o.y, o.z, c.x, c.y, c.z);
@@ -658,9 +656,9 @@ template <int K, int M, int P, bool kAll>
 __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
     int leaf_size, int init_leaves, const uint32_t* __restrict__ wleaf, const uint32_t* __restrict__ wleaf_n,
-    const float* __restrict__ tau_sorted, const float* __restrict__ hinv_sorted, float* __restrict__ rk_sorted,
-    uint64_t* __restrict__ kkey_sorted, double* __restrict__ dens64, double* __restrict__ dens_pull,
-    int32_t* __restrict__ nbr_out, uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats)
+    const float* __restrict__ tau_sorted, float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted,
+    double* __restrict__ dens64, double* __restrict__ dens_pull, int32_t* __restrict__ nbr_out,
+    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats)
 {
     __shared__ CollectLds<M> lds[kCollectWaves];
     const int lane = threadIdx.x % kWave;
@@ -723,14 +721,12 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
             int ncount = __builtin_amdgcn_readlane(mb.hi_w, nxt);
             float4 cn = (j < ncount) ? spos[nstart + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
             float tn = (j < ncount) ? tau_sorted[nstart + j] : 0.0f;
-            float hn = (j < ncount) ? hinv_sorted[nstart + j] : 0.0f;
             while (part != 0) {
                 const int cur = nxt;
                 const int start = nstart;
                 const int count = ncount;
                 const float4 c = cn;
                 const float tc = tn;
-                const float hc = hn;
                 part &= part - 1;
                 if (part != 0) {
                     nxt = __builtin_ctzll(part);
@@ -738,12 +734,11 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
                     ncount = __builtin_amdgcn_readlane(mb.hi_w, nxt);
                     cn = (j < ncount) ? spos[nstart + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
                     tn = (j < ncount) ? tau_sorted[nstart + j] : 0.0f;
-                    hn = (j < ncount) ? hinv_sorted[nstart + j] : 0.0f;
                 }
                 const uint64_t mask = (static_cast<uint64_t>(__builtin_amdgcn_readlane(m_hi, cur)) << 32) |
                                       static_cast<uint32_t>(__builtin_amdgcn_readlane(m_lo, cur));
                 ++n_leaf;
-                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, hc, lane, qx, qy, qz, tau, hinv, dens64,
+                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64,
                                          n_pairs);
             }
         }
@@ -763,9 +758,7 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
                 const int count = b.hi_w;
                 const float4 c = (j < count) ? spos[start + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
                 const float tc = (j < count) ? tau_sorted[start + j] : 0.0f;
-                const float hc = (j < count) ? hinv_sorted[start + j] : 0.0f;
-                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, hc, lane, qx, qy, qz, tau, hinv, dens64,
-                                         n_pairs);
+                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64, n_pairs);
             } else {
                 stack = (lane == sp) ? 2 * node + 1 : stack;
                 ++sp;
@@ -957,13 +950,13 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     mark_begin(0);
     knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n, stats,
-        scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks, groups, scratch.hinv);
+        scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks, groups);
     ASM_HIP_CHECK(hipGetLastError());
     if (scratch.tau_init) {
         // Wavefronts with a lane whose learned bound was too small: again, unbounded.
         knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
             spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
-            stats, nullptr, scratch.seed_redo, nullptr, nullptr, groups, scratch.hinv);
+            stats, nullptr, scratch.seed_redo, nullptr, nullptr, groups);
         ASM_HIP_CHECK(hipGetLastError());
     }
     mark_end(0);
@@ -972,7 +965,6 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
 #define ASM_COLLECT(M, PAD, ALL)                                                                                \
     knn_collect_kernel<kSlots, M, PAD, ALL><<<col_blocks, kCollectThreads, 0, s>>>(                             \
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.wleaf, scratch.wleaf_n, scratch.tau,  \
-        scratch.hinv,                                                                                          \
         rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats)
 #define ASM_COLLECT_PAD(M, ALL)                  \
     if (leaf_size <= 8) ASM_COLLECT(M, 8, ALL);  \
