@@ -476,18 +476,15 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
                 // the two-level path below instead.
                 const int g0 = 4 * node;
                 const int la = g0 - nlp;
-                // The four boxes are 128 contiguous bytes: loaded together,
-                // tested without branches.
+                uint32_t mask = 0, seeded = 0;
                 Box lb[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) lb[j] = nodes[g0 + j];
-                uint32_t need = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) need |= wave_any(box_d2(lb[j], qx, qy, qz) <= kth) ? (1u << j) : 0u;
-                const int slo = l0 - la > 0 ? l0 - la : 0;
-                const int shi = l1 - la < 3 ? l1 - la : 3;
-                const uint32_t seeded = slo <= shi ? (2u << shi) - (1u << slo) : 0u;
-                const uint32_t mask = need & ~seeded;
+                for (int j = 0; j < 4; ++j) {
+                    lb[j] = nodes[g0 + j];
+                    const bool sj = la + j >= l0 && la + j <= l1;
+                    seeded |= sj ? (1u << j) : 0u;
+                    if (!sj && wave_any(box_d2(lb[j], qx, qy, qz) <= kth)) mask |= 1u << j;
+                }
                 if (mask == 0u) continue;
                 const int jlo = __builtin_ctz(mask), jhi = 31 - __builtin_clz(mask);
                 const uint32_t span = (2u << jhi) - (1u << jlo);
