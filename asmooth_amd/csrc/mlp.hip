@@ -1,9 +1,9 @@
 // Learned seed radius: a 27-64-64-1 multilayer perceptron (GELU) that maps
 // ln(CIC density) over the 3x3x3 cells around a particle, on a 64^3 mesh of
 // the tree cube, to ln(h / s), s the mean particle spacing of the set.  It
-// has no counterpart in the reference; the exact search stays the judge (a
-// predicted radius only bounds the search, and a lane whose radius holds
-// fewer than 33 particles is searched again without it).
+// has no counterpart in the reference, and the exact search still decides
+// every result (a predicted radius only bounds the search, and a lane whose
+// radius holds fewer than 33 particles is searched again without it).
 //
 // Training runs on the device with these kernels: a per-sample forward pass
 // that keeps the pre-activations, a per-sample backward pass to the
