@@ -254,9 +254,9 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
 // ---------------------------------------------------------------------------
 // Bound walk and collect walk.
 
-constexpr int kBoundThreads = 256;
+constexpr int kBoundThreads = 128;
 constexpr int kBoundWaves = kBoundThreads / kWave;
-constexpr int kCollectThreads = 128;
+constexpr int kCollectThreads = 64;
 constexpr int kCollectWaves = kCollectThreads / kWave;
 constexpr int kCollectSlots = 40;   // per-lane list of all entries: 33 plus up to 7 ties at tau
 constexpr int kTieSlots = 8;        // per-lane list of the entries at tau
