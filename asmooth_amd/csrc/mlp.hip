@@ -144,36 +144,107 @@ __global__ __launch_bounds__(256) void mlp_backward_kernel(const float* __restri
     }
 }
 
-// One thread per parameter: its gradient as a sum over the batch, in order.
-// h1, h2 are the activations (see mlp_backward_kernel).
-__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float* __restrict__ x, const int64_t* __restrict__ idx,
+// Weight gradients as small GEMMs on the matrix cores.  Over the batch b:
+//   dW1[k][i] = sum x[b][k] dz1[b][i],  db1[i] = sum dz1[b][i]      (rows k < 27, row 27 = ones)
+//   dW2[i][j] = sum h1[b][i] dz2[b][j], db2[j] = sum dz2[b][j]      (rows i < 64, row 64 = ones)
+//   dw3[j]    = sum h2[b][j] dy[b],     db3    = sum dy[b]          (rows j < 64, row 64 = ones)
+// Output blocks of 32 x 32: 0-1 the first layer (one row block, two column
+// blocks), 2-7 the second (three row blocks, two column blocks), 8-10 the
+// output layer (three row blocks, one column block whose column 0 is dy).
+// A block's batch is cut into kWgradSplit slices; one wavefront per (block,
+// slice) runs v_mfma_f32_32x32x2_f32 over its slice, two samples per step,
+// and stores its partial block; mlp_wgrad_reduce_kernel adds the slices in
+// order (deterministic run to run).  Operands: lane l holds A[l % 32][l / 32]
+// (row, sample) and B[l / 32][l % 32] (sample, column); lane l ends with
+// D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32] in accumulator v.  h1, h2 are the
+// activations (see mlp_backward_kernel).
+typedef float v16f __attribute__((ext_vector_type(16)));
+constexpr int kWgradBlocks = 11;
+constexpr int kWgradSplit = 32;
+constexpr int kWgradPartial = kWgradBlocks * kWgradSplit * kWave * 16;   // floats
+
+struct WgradBlock {
+    int layer, mb, nbk;
+};
+
+__device__ __forceinline__ WgradBlock wgrad_block(int blk)
+{
+    WgradBlock w;
+    w.layer = blk < 2 ? 1 : (blk < 8 ? 2 : 3);
+    w.mb = w.layer == 1 ? 0 : (w.layer == 2 ? (blk - 2) / 2 : blk - 8);
+    w.nbk = w.layer == 1 ? blk : (w.layer == 2 ? (blk - 2) % 2 : 0);
+    return w;
+}
+
+__global__ __launch_bounds__(kWave) void mlp_wgrad_kernel(const float* __restrict__ x, const int64_t* __restrict__ idx,
                                                         int64_t nb, const float* __restrict__ h1,
                                                         const float* __restrict__ h2, const float* __restrict__ dz1,
                                                         const float* __restrict__ dz2, const float* __restrict__ dy,
-                                                        float* __restrict__ grad)
+                                                        float* __restrict__ partial)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpParams) return;
-    double s = 0.0;
-    if (p < kOffB1) {
-        const int k = p / kH, i = p % kH;
-        for (int64_t b = 0; b < nb; ++b) s += static_cast<double>(x[(idx ? idx[b] : b) * kIn + k]) * dz1[b * kH + i];
-    } else if (p < kOffW2) {
-        const int i = p - kOffB1;
-        for (int64_t b = 0; b < nb; ++b) s += dz1[b * kH + i];
-    } else if (p < kOffB2) {
-        const int i = (p - kOffW2) / kH, j = (p - kOffW2) % kH;
-        for (int64_t b = 0; b < nb; ++b) s += static_cast<double>(h1[b * kH + i]) * dz2[b * kH + j];
-    } else if (p < kOffW3) {
-        const int j = p - kOffB2;
-        for (int64_t b = 0; b < nb; ++b) s += dz2[b * kH + j];
-    } else if (p < kOffB3) {
-        const int j = p - kOffW3;
-        for (int64_t b = 0; b < nb; ++b) s += static_cast<double>(h2[b * kH + j]) * dy[b];
-    } else {
-        for (int64_t b = 0; b < nb; ++b) s += dy[b];
+    const int lane = threadIdx.x;
+    const int blk = blockIdx.x / kWgradSplit;
+    const int slice = blockIdx.x % kWgradSplit;
+    const WgradBlock w = wgrad_block(blk);
+    const int row = w.mb * 32 + lane % 32;   // this lane's A row
+    const int col = w.nbk * 32 + lane % 32;  // this lane's B column
+    const int ks = lane / 32;                // sample within the step
+    const int nrows = w.layer == 1 ? kIn : kH;   // rows past nrows: the ones row, then zeros
+    // Slice bounds: an even number of samples per slice.
+    const int64_t per = ((nb + kWgradSplit - 1) / kWgradSplit + 1) & ~static_cast<int64_t>(1);
+    const int64_t s0 = slice * per;
+    const int64_t s1 = s0 + per < nb ? s0 + per : nb;
+    v16f acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    for (int64_t b0 = s0; b0 < s1; b0 += 2) {
+        const int64_t b = b0 + ks;
+        float av = 0.0f, bv = 0.0f;
+        if (b < s1) {
+            if (row < nrows) {
+                av = w.layer == 1 ? x[(idx ? idx[b] : b) * kIn + row]
+                                  : (w.layer == 2 ? h1[b * kH + row] : h2[b * kH + row]);
+            } else if (row == nrows) {
+                av = 1.0f;
+            }
+            if (w.layer == 1) bv = dz1[b * kH + col];
+            else if (w.layer == 2) bv = dz2[b * kH + col];
+            else bv = col == 0 ? dy[b] : 0.0f;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
     }
-    grad[p] = static_cast<float>(s);
+    float* const out = partial + (static_cast<int64_t>(blk) * kWgradSplit + slice) * (kWave * 16) + lane * 16;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) out[v] = acc[v];
+}
+
+// One thread per (block, lane, accumulator): the slices summed in order,
+// stored at the parameter the element stands for.
+__global__ void mlp_wgrad_reduce_kernel(const float* __restrict__ partial, float* __restrict__ grad)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kWgradBlocks * kWave * 16) return;
+    const int blk = e / (kWave * 16);
+    const int lane = (e / 16) % kWave;
+    const int v = e % 16;
+    float sum = 0.0f;
+    for (int sl = 0; sl < kWgradSplit; ++sl)
+        sum += partial[(static_cast<int64_t>(blk) * kWgradSplit + sl) * (kWave * 16) + lane * 16 + v];
+    const WgradBlock w = wgrad_block(blk);
+    const int r = w.mb * 32 + 8 * (v / 4) + 4 * (lane / 32) + v % 4;
+    const int col = w.nbk * 32 + lane % 32;
+    int p = -1;
+    if (w.layer == 1) {
+        if (r < kIn) p = kOffW1 + r * kH + col;
+        else if (r == kIn) p = kOffB1 + col;
+    } else if (w.layer == 2) {
+        if (r < kH) p = kOffW2 + r * kH + col;
+        else if (r == kH) p = kOffB2 + col;
+    } else if (col == 0) {
+        if (r < kH) p = kOffW3 + r;
+        else if (r == kH) p = kOffB3;
+    }
+    if (p >= 0) grad[p] = sum;
 }
 
 __global__ void adam_kernel(float* __restrict__ params, const float* __restrict__ grad, float* __restrict__ m,
@@ -459,13 +530,14 @@ int mlp_fail(asm_mlp* m, int code, const std::string& msg)
 // Device scratch of one batch.
 struct Batch {
     float *x = nullptr, *t = nullptr, *y = nullptr, *z1 = nullptr, *z2 = nullptr, *dz1 = nullptr, *dz2 = nullptr,
-          *dy = nullptr, *loss = nullptr;
+          *dy = nullptr, *loss = nullptr, *wpart = nullptr;
     int64_t* idx = nullptr;
     ~Batch()
     {
         for (void* p : {static_cast<void*>(x), static_cast<void*>(t), static_cast<void*>(y), static_cast<void*>(z1),
                         static_cast<void*>(z2), static_cast<void*>(dz1), static_cast<void*>(dz2),
-                        static_cast<void*>(dy), static_cast<void*>(loss), static_cast<void*>(idx)})
+                        static_cast<void*>(dy), static_cast<void*>(loss), static_cast<void*>(idx),
+                        static_cast<void*>(wpart)})
             if (p) (void)hipFree(p);
     }
     hipError_t alloc(int64_t nrows, int64_t nb)
@@ -480,6 +552,7 @@ struct Batch {
         if ((e = hipMalloc(reinterpret_cast<void**>(&dz2), sizeof(float) * nb * kH)) != hipSuccess) return e;
         if ((e = hipMalloc(reinterpret_cast<void**>(&dy), sizeof(float) * nb)) != hipSuccess) return e;
         if ((e = hipMalloc(reinterpret_cast<void**>(&loss), sizeof(float) * nb)) != hipSuccess) return e;
+        if ((e = hipMalloc(reinterpret_cast<void**>(&wpart), sizeof(float) * kWgradPartial)) != hipSuccess) return e;
         return hipMalloc(reinterpret_cast<void**>(&idx), sizeof(int64_t) * nb);
     }
 };
@@ -494,8 +567,11 @@ hipError_t batch_grad(asm_mlp* m, Batch& B, const int64_t* idx, int64_t nb)
     mlp_backward_kernel<<<blocks, 256, 0, m->stream>>>(m->params, B.t, idx, nb, B.y, B.z1, B.z2, B.dz1, B.dz2, B.dy,
                                                        B.loss);
     ASM_HIP_CHECK(hipGetLastError());
-    mlp_wgrad_kernel<<<(kMlpParams + 255) / 256, 256, 0, m->stream>>>(B.x, idx, nb, B.z1, B.z2, B.dz1, B.dz2, B.dy,
-                                                                       m->grad);
+    mlp_wgrad_kernel<<<kWgradBlocks * kWgradSplit, kWave, 0, m->stream>>>(B.x, idx, nb, B.z1, B.z2, B.dz1, B.dz2,
+                                                                         B.dy, B.wpart);
+    ASM_HIP_CHECK(hipGetLastError());
+    constexpr int kRed = kWgradBlocks * kWave * 16;
+    mlp_wgrad_reduce_kernel<<<(kRed + 255) / 256, 256, 0, m->stream>>>(B.wpart, m->grad);
     return hipGetLastError();
 }
 
