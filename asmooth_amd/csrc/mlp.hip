@@ -270,8 +270,18 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
 // kCicWin^3 cells placed at the block's lowest stencil corner and adds the
 // window's non-zero cells to the mesh; stencils outside the window (a block
 // spanning a Morton jump) go to the mesh directly.
+// CIC counts on the kMlpGrid^3 mesh.  A workgroup takes kCicThreads x
+// kCicRun Morton-consecutive particles, which fall in a few cells: it adds
+// into an LDS window of kCicWin^3 cells at the smallest cell it touches,
+// then adds the window's nonzero cells to the mesh (global atomics).  Each
+// thread walks kCicRun consecutive particles and keeps the eight corner
+// weights of the current cell in registers until the cell changes (tens of
+// consecutive particles share a cell), so the LDS sees one set of eight
+// atomics per run of a cell instead of one per particle.  Corners outside
+// the window go straight to the mesh; corners past the mesh are clamped onto
+// its edge cells.
 constexpr int kCicThreads = 256;
-constexpr int kCicPerThread = 4;
+constexpr int kCicRun = 16;
 constexpr int kCicWin = 12;
 
 __device__ __forceinline__ void cic_stencil(float4 p, float left, float inv_cell, int (&c0)[3], float (&w1)[3])
@@ -285,55 +295,88 @@ __device__ __forceinline__ void cic_stencil(float4 p, float left, float inv_cell
     }
 }
 
+// The eight corner weights of stencil w1 (x first), added to acc.
+__device__ __forceinline__ void cic_add(const float (&w1)[3], float (&acc)[8])
+{
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx)
+                acc[(dz * 2 + dy) * 2 + dx] += (dx ? w1[0] : 1.0f - w1[0]) * (dy ? w1[1] : 1.0f - w1[1]) *
+                                               (dz ? w1[2] : 1.0f - w1[2]);
+}
+
+// The corners of cell c0 with weights acc: into the window when they fit,
+// else into the mesh.
+__device__ __forceinline__ void cic_flush(const int (&c0)[3], const float (&acc)[8], const int (&k0)[3], float* win,
+                                          int nf, float* __restrict__ grid)
+{
+    const bool inwin = c0[0] >= k0[0] && c0[1] >= k0[1] && c0[2] >= k0[2] && c0[0] + 1 - k0[0] < kCicWin &&
+                       c0[1] + 1 - k0[1] < kCicWin && c0[2] + 1 - k0[2] < kCicWin;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const float w = acc[(dz * 2 + dy) * 2 + dx];
+                if (inwin) {
+                    atomicAdd(&win[((c0[2] + dz - k0[2]) * kCicWin + (c0[1] + dy - k0[1])) * kCicWin +
+                                   (c0[0] + dx - k0[0])],
+                              w);
+                } else {
+                    const int cx = min(max(c0[0] + dx, 0), nf - 1);
+                    const int cy = min(max(c0[1] + dy, 0), nf - 1);
+                    const int cz = min(max(c0[2] + dz, 0), nf - 1);
+                    atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], w);
+                }
+            }
+}
+
 __global__ __launch_bounds__(kCicThreads) void cic_kernel(const float4* __restrict__ spos, int64_t n, float left,
                                                           float inv_cell, int nf, float* __restrict__ grid)
 {
     __shared__ float win[kCicWin * kCicWin * kCicWin];
     __shared__ int corner[3];
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kCicThreads * kCicPerThread;
+    const int64_t first = (static_cast<int64_t>(blockIdx.x) * kCicThreads + threadIdx.x) * kCicRun;
+    const int cnt = static_cast<int>(first >= n ? 0 : (n - first < kCicRun ? n - first : kCicRun));
     for (int c = threadIdx.x; c < kCicWin * kCicWin * kCicWin; c += kCicThreads) win[c] = 0.0f;
     if (threadIdx.x < 3) corner[threadIdx.x] = 0x7fffffff;
     __syncthreads();
-    int c0[kCicPerThread][3];
-    float w1[kCicPerThread][3];
-    bool act[kCicPerThread];
+    // Window corner: the smallest cell any particle of the workgroup touches.
+    int m0[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+    for (int r = 0; r < cnt; ++r) {
+        int c0[3];
+        float w1[3];
+        cic_stencil(spos[first + r], left, inv_cell, c0, w1);
 #pragma unroll
-    for (int r = 0; r < kCicPerThread; ++r) {
-        const int64_t i = base + static_cast<int64_t>(r) * kCicThreads + threadIdx.x;
-        act[r] = i < n;
-        if (act[r]) {
-            cic_stencil(spos[i], left, inv_cell, c0[r], w1[r]);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) atomicMin(&corner[d], c0[r][d]);
-        }
+        for (int d = 0; d < 3; ++d) m0[d] = min(m0[d], c0[d]);
     }
+    if (cnt > 0)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) atomicMin(&corner[d], m0[d]);
     __syncthreads();
     const int k0[3] = {corner[0], corner[1], corner[2]};
+    int cur[3] = {0, 0, 0};
+    float acc[8];
 #pragma unroll
-    for (int r = 0; r < kCicPerThread; ++r) {
-        if (!act[r]) continue;
-        const bool inwin = c0[r][0] + 1 - k0[0] < kCicWin && c0[r][1] + 1 - k0[1] < kCicWin &&
-                           c0[r][2] + 1 - k0[2] < kCicWin;
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+    for (int r = 0; r < cnt; ++r) {
+        int c0[3];
+        float w1[3];
+        cic_stencil(spos[first + r], left, inv_cell, c0, w1);
+        if (r > 0 && (c0[0] != cur[0] || c0[1] != cur[1] || c0[2] != cur[2])) {
+            cic_flush(cur, acc, k0, win, nf, grid);
 #pragma unroll
-        for (int dz = 0; dz < 2; ++dz)
+            for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+        }
 #pragma unroll
-            for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                    const float w = (dx ? w1[r][0] : 1.0f - w1[r][0]) * (dy ? w1[r][1] : 1.0f - w1[r][1]) *
-                                    (dz ? w1[r][2] : 1.0f - w1[r][2]);
-                    if (inwin) {
-                        atomicAdd(&win[((c0[r][2] + dz - k0[2]) * kCicWin + (c0[r][1] + dy - k0[1])) * kCicWin +
-                                       (c0[r][0] + dx - k0[0])],
-                                  w);
-                    } else {
-                        const int cx = min(max(c0[r][0] + dx, 0), nf - 1);
-                        const int cy = min(max(c0[r][1] + dy, 0), nf - 1);
-                        const int cz = min(max(c0[r][2] + dz, 0), nf - 1);
-                        atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], w);
-                    }
-                }
+        for (int d = 0; d < 3; ++d) cur[d] = c0[d];
+        cic_add(w1, acc);
     }
+    if (cnt > 0) cic_flush(cur, acc, k0, win, nf, grid);
     __syncthreads();
     for (int c = threadIdx.x; c < kCicWin * kCicWin * kCicWin; c += kCicThreads) {
         const float v = win[c];
@@ -464,7 +507,7 @@ hipError_t launch_seed_grid(const float4* spos, int64_t n, float left, float rig
     const int64_t ncell = static_cast<int64_t>(nf) * nf * nf;
     ASM_HIP_CHECK(launch_fill_zero_f32(grid, ncell, s));
     const float inv_cell = static_cast<float>(nf) / (right - left);
-    const int64_t per_block = static_cast<int64_t>(kCicThreads) * kCicPerThread;
+    const int64_t per_block = static_cast<int64_t>(kCicThreads) * kCicRun;
     if (n > 0)
         cic_kernel<<<static_cast<unsigned>((n + per_block - 1) / per_block), kCicThreads, 0, s>>>(spos, n, left,
                                                                                                  inv_cell, nf, grid);

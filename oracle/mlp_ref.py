@@ -13,7 +13,11 @@ trained on the mean squared error.  Pins (tests/test_mlp_oracle.py):
   * forward: g(0) = 0, g'(0) = 1/2, g against the erf form of GELU (the
     tanh form is within 1e-3 of it everywhere), and a network with one
     active path reduces to a hand-computed composition;
-  * gradients: central finite differences of the loss in float64.
+  * gradients: central finite differences of the loss in float64;
+  * features (seed_features): the CIC mesh conserves mass, a particle at a
+    cell centre lands in one cell and one on a cell corner splits evenly
+    over eight, and the 27 features of a particle are its cell's
+    neighbourhood in (dz, dy, dx) order with x fastest.
 """
 from __future__ import annotations
 
@@ -81,3 +85,61 @@ def loss_grad(p, x, t):
     gb1 = dz1.sum(0)
     g = np.concatenate([gw1.ravel(), gb1, gw2.ravel(), gb2, gw3, [gb3]])
     return loss, g
+
+
+N_GRID = 64
+LN_FLOOR = 1e-3
+
+
+def cic_grid(pos, left: float, right: float, nf: int = N_GRID):
+    """CIC counts of the particles on an nf^3 mesh of [left, right)^3, corners
+    past the mesh clamped onto its edge cells.  The stencil is evaluated in
+    float32 as the device does (u = (x - left) * (nf / (right - left)) - 1/2,
+    corner weights as float32 products, x first); the counts are summed in
+    float64."""
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    left32 = np.float32(left)
+    inv_cell = np.float32(nf) / (np.float32(right) - left32)
+    u = (pos - left32) * inv_cell - np.float32(0.5)
+    c0 = np.floor(u).astype(np.int64)
+    w1 = (u - np.floor(u)).astype(np.float32)
+    one = np.float32(1.0)
+    grid = np.zeros(nf ** 3, np.float64)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                wx = w1[:, 0] if dx else one - w1[:, 0]
+                wy = w1[:, 1] if dy else one - w1[:, 1]
+                wz = w1[:, 2] if dz else one - w1[:, 2]
+                w = (wx * wy) * wz
+                cx = np.clip(c0[:, 0] + dx, 0, nf - 1)
+                cy = np.clip(c0[:, 1] + dy, 0, nf - 1)
+                cz = np.clip(c0[:, 2] + dz, 0, nf - 1)
+                np.add.at(grid, (cz * nf + cy) * nf + cx, w.astype(np.float64))
+    return grid
+
+
+def seed_features(pos, left: float, right: float, idx=None, nf: int = N_GRID):
+    """Features (m, 27) of particles idx: ln(CIC count / mean + 1e-3) over the
+    3 x 3 x 3 cells around each particle's cell (clamped to the mesh), in
+    (dz, dy, dx) order with dx fastest."""
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    n = len(pos)
+    grid = cic_grid(pos, left, right, nf)
+    lng = np.log(grid * (float(nf ** 3) / n) + LN_FLOOR)
+    q = pos if idx is None else pos[np.asarray(idx)]
+    left32 = np.float32(left)
+    inv_cell = np.float32(nf) / (np.float32(right) - left32)
+    c = np.clip(np.floor((q - left32) * inv_cell).astype(np.int64), 0, nf - 1)
+    out = np.empty((len(q), 27), np.float64)
+    k = 0
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                gx = np.clip(c[:, 0] + dx, 0, nf - 1)
+                gy = np.clip(c[:, 1] + dy, 0, nf - 1)
+                gz = np.clip(c[:, 2] + dz, 0, nf - 1)
+                out[:, k] = lng[(gz * nf + gy) * nf + gx]
+                k += 1
+    return out
+

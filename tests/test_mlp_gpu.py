@@ -89,6 +89,9 @@ def test_seed_samples_and_trained_model(ctx):
     s = (side ** 3 / len(parts)) ** (1.0 / 3.0)
     assert np.allclose(t, np.log(plain["rk"] / np.float32(s)), rtol=0, atol=1e-5)
     assert np.all(np.isfinite(x)) and x.shape == (len(parts), 27)
+    # features against the numpy restatement (FP32 sums on the device)
+    ox = M.seed_features(parts["x"], -bf * L, (1 + bf) * L)
+    assert np.max(np.abs(x - ox)) < 1e-4
     m = A.Mlp(0, A.mlp_init_params(0))
     m.train(x, t, epochs=30, batch=512, lr=3e-3, seed=2)
     ctx.set_seed_model(m, 0.25)

@@ -49,3 +49,35 @@ def test_gradient_matches_finite_differences():
         pm = p.copy(); pm[i] -= h
         fd = (M.loss_grad(pp, x, t)[0] - M.loss_grad(pm, x, t)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_cic_grid_closed_forms():
+    nf = 8
+    # cell centre: the whole particle in one cell
+    g = M.cic_grid(np.array([[2.5, 3.5, 4.5]]), 0.0, 8.0, nf)
+    assert g.sum() == 1.0 and g[(4 * nf + 3) * nf + 2] == 1.0
+    # cell corner: an eighth in each of the eight cells around it
+    g = M.cic_grid(np.array([[3.0, 3.0, 3.0]]), 0.0, 8.0, nf)
+    for cz in (2, 3):
+        for cy in (2, 3):
+            for cx in (2, 3):
+                assert g[(cz * nf + cy) * nf + cx] == 0.125
+    # mass conservation, including particles near the edges (clamped)
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-0.2, 8.2, (5000, 3))
+    assert abs(M.cic_grid(pos, 0.0, 8.0, nf).sum() - 5000.0) < 1e-6
+
+
+def test_seed_features_neighbourhood_order():
+    nf = 8
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0.0, 8.0, (3000, 3)).astype(np.float32)
+    f = M.seed_features(pos, 0.0, 8.0, idx=[0, 1], nf=nf)
+    lng = np.log(M.cic_grid(pos, 0.0, 8.0, nf) * (nf ** 3 / 3000) + M.LN_FLOOR)
+    for r in range(2):
+        c = np.clip(np.floor(pos[r] * np.float32(nf / 8.0)).astype(int), 0, nf - 1)
+        # dz = +1, dy = -1, dx = 0 is feature (2 * 3 + 0) * 3 + 1 = 19
+        g = np.clip(c + np.array([0, -1, 1]), 0, nf - 1)
+        assert f[r, 19] == lng[(g[2] * nf + g[1]) * nf + g[0]]
+        assert f[r, 13] == lng[(c[2] * nf + c[1]) * nf + c[0]]   # the particle's own cell
+
