@@ -22,6 +22,7 @@ import asmooth_amd as A
 from asmooth_amd import synth
 
 COPY_BYTES = 1 << 30
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector rate
 
 
 def _short(name: str) -> str:
@@ -110,10 +111,55 @@ def summarize_sq(counters_csv: str, command: str) -> str:
     return "\n".join(out) + "\n"
 
 
+def summarize_busy(counters_csv: str, stats_csv: str, command: str, json_out: str | None = None) -> str:
+    """Markdown table of rocprofv3's derived occupancy, VALU-lane utilisation,
+    SALU-busy and LDS bank-conflict percentages per kernel (mean over its
+    dispatches), and its FP32 vector arithmetic: SQ_INSTS_VALU_FLOPS_FP32
+    counts the FLOPs of one lane per wavefront instruction (FMA = 2), so a
+    dispatch performs 64 times that; over the mean duration of the kernel in
+    the kernel-trace run it gives the achieved FP32 rate.  With json_out, the
+    FLOPs per dispatch as JSON (kernel -> fp32_flops)."""
+    import csv
+    import collections
+    import json
+
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(counters_csv)):
+        vals[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    avg_ms = {_short(r["Name"]): float(r["AverageNs"]) / 1e6 for r in csv.DictReader(open(stats_csv))}
+    mean = lambda xs: sum(xs) / len(xs) if xs else 0.0
+    out = ["Command (from the repository root, TMPDIR=/tmp):", "", f"    {command}", "",
+           "Percentages are rocprofv3's derived OccupancyPercent, VALUUtilization, SALUBusy and "
+           "LDSBankConflict, averaged over a kernel's dispatches.  FP32 GFLOP/launch = 64 x "
+           "SQ_INSTS_VALU_FLOPS_FP32; TFLOP/s uses the kernel's mean duration in the kernel-trace run "
+           f"(profiles/traffic_zeldovich256.md); peak FP32 vector rate {FP32_PEAK_TFLOPS} TFLOP/s.", "",
+           "| kernel | occupancy | VALU lanes active | SALU busy | LDS bank conflict | FP32 GFLOP/launch | "
+           "TFLOP/s | of FP32 peak |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    js = {"fp32_peak_tflops": FP32_PEAK_TFLOPS, "kernels": {}}
+    rows = sorted(vals.items(), key=lambda kv: -avg_ms.get(kv[0], 0.0) * len(kv[1].get("OccupancyPercent", [])))
+    for k, c in rows:
+        flops = 64.0 * mean(c.get("SQ_INSTS_VALU_FLOPS_FP32", []))
+        ms = avg_ms.get(k)
+        rate = flops / (ms * 1e-3) / 1e12 if ms else None
+        rs = f"{rate:.2f} | {100.0 * rate / FP32_PEAK_TFLOPS:.1f}%" if rate is not None else "- | -"
+        out.append(f"| {k} | {mean(c.get('OccupancyPercent', [])):.0f}% | {mean(c.get('VALUUtilization', [])):.0f}% | "
+                   f"{mean(c.get('SALUBusy', [])):.0f}% | {mean(c.get('LDSBankConflict', [])):.1f}% | "
+                   f"{flops / 1e9:.1f} | {rs} |")
+        js["kernels"][k] = {"fp32_flops": flops, "avg_ms": ms}
+    if json_out:
+        with open(json_out, "w") as f:
+            json.dump(js, f, indent=1)
+    return "\n".join(out) + "\n"
+
+
 def main() -> int:
     if len(sys.argv) > 1 and sys.argv[1] == "sq":
         # sq <counters.csv> "<command>"
         print(summarize_sq(sys.argv[2], sys.argv[3]))
+        return 0
+    if len(sys.argv) > 1 and sys.argv[1] == "busy":
+        # busy <counters.csv> <stats.csv> "<command>" [json_out]
+        print(summarize_busy(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5] if len(sys.argv) > 5 else None))
         return 0
     if len(sys.argv) > 1 and sys.argv[1] == "summarize":
         # summarize <stats.csv> <fetch.csv> <write.csv> "<cmd1> ;; <cmd2> ;; ..." [json_out]

@@ -46,6 +46,8 @@ def log(msg: str) -> None:
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_zeldovich256.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_zeldovich256.json")
+FP32_PEAK_TFLOPS = 157.3
 
 
 def measured_traffic(kernel: str):
@@ -65,6 +67,18 @@ def measured_traffic(kernel: str):
         return None
     k = max(cands, key=lambda v: v.get("calls", 0))
     return int(k["read_bytes"] + k["write_bytes"])
+
+
+def measured_fp32_flops(kernel: str):
+    """FP32 vector FLOPs per launch of `kernel` (64 x SQ_INSTS_VALU_FLOPS_FP32)
+    from the committed counter summary of this workload, or None."""
+    try:
+        with open(VALU_FILE) as f:
+            kernels = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    cands = [v["fp32_flops"] for name, v in kernels.items() if name.split("<")[0] == kernel and v.get("fp32_flops")]
+    return max(cands) if cands else None
 
 
 def make_workload(n_side: int, L: float, bf: float, seed: int):
@@ -292,6 +306,12 @@ def main(argv=None) -> int:
     roofline = {"bound": "hbm", "kernel": alg[dom][1], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                 "launch_ms": round(dom_ms, 4), "bytes_per_launch": int(alg[dom][0]), "note": alg[dom][2]}
+    flops = measured_fp32_flops(alg[dom][1])
+    if flops:
+        # the same launch against the FP32 vector rate (counted FLOPs, live duration)
+        tf = flops / (dom_ms * 1e-3) / 1e12
+        roofline["fp32_vector"] = {"achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": round(tf / FP32_PEAK_TFLOPS, 4), "flops_per_launch": int(flops)}
 
     b_alg = 80 * nbuf + 24 * sum(nc ** 3 for nc in ncs)
     mass_ok = all(0.999 < (t[0] + t[1]) / nl < 1.001 for t in totals)

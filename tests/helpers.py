@@ -84,3 +84,64 @@ def rel_err(a: np.ndarray, b: np.ndarray, floor: float = 0.0) -> float:
 def mesh_fields(mesh: np.ndarray, nc: int):
     m = nc ** 3
     return dict(n=mesh[:m], halo=mesh[m:2 * m], c=mesh[2 * m:3 * m], v=mesh[3 * m:6 * m])
+
+
+def region_oracle(buf: np.ndarray, L: float, center, half: float, hmax: float, ncs, sign: int = -1):
+    """Oracle results for one cubic region of a buffered set too large for the
+    oracle to process whole.
+
+    Queries are the particles within `half + m1` (infinity norm) of `center`,
+    with m1 = hmax + the largest cell size + 1; candidates are those within
+    `half + m2`, m2 = m1 + 2 hmax + 1.  The oracle's kNN over the candidates
+    equals its kNN over the whole set for every particle whose distance to the
+    candidate cube's faces exceeds its own h (no particle outside lies within
+    h); that is asserted for every particle up to `half + m1 + hmax`, so the
+    queries' h and neighbour lists are exact, and so are their densities,
+    which collect the tent terms of neighbours within hmax (orc_density
+    scatters each particle's terms onto its neighbours).  Cells lying inside
+    the cube of half-width `half` receive terms only from the queries, so the
+    oracle deposition of the queries gives those cells exactly.
+
+    Returns the queries' indices into `buf`, their rk, neighbour lists (indices
+    into `buf`) and FP64 densities, and per nc the flat cell indices of the
+    block and the oracle's six FP64 fields there (n, halo, c, vx, vy, vz)."""
+    pos = np.ascontiguousarray(buf[:, :3], np.float32)
+    vel = np.ascontiguousarray(buf[:, 3:6], np.float32)
+    c = np.asarray(center, np.float64)
+    m1 = hmax + max(L / nc for nc in ncs) + 1.0
+    m2 = m1 + 2.0 * hmax + 1.0
+    dist = np.max(np.abs(pos.astype(np.float64) - c), axis=1)
+    cand = np.nonzero(dist < half + m2)[0]
+    p, v, dc = pos[cand], vel[cand], dist[cand]
+    nbr, _, rk = O.knn(p)
+    core = dc < half + m1 + hmax
+    assert np.all(rk[core] < (half + m2) - dc[core]), "candidate cube too small for these h"
+    assert float(rk[core].max()) <= hmax
+    dens64 = O.density(p, nbr, rk)
+    q = dc < half + m1
+    out = dict(index=cand[q], rk=rk[q], nbr=cand[nbr[q]], dens64=dens64[q], cells={})
+    dens32 = dens64.astype(np.float32)
+    igrp = np.zeros(int(q.sum()), np.int32)
+    for nc in ncs:
+        dx = L / nc
+        ax = []
+        for a in range(3):
+            i0 = max(0, int(np.ceil((c[a] - half) / dx)))
+            i1 = min(nc, int(np.floor((c[a] + half) / dx)))
+            ax.append(np.arange(i0, max(i0, i1), dtype=np.int64))
+        iz, iy, ix = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        cells = ((iz * nc + iy) * nc + ix).ravel()
+        om = O.assign_on_mesh(p[q], v[q], dens32[q], rk[q], igrp, L, nc, sign)
+        m = nc ** 3
+        fields = np.stack([om[cells], om[m + cells], om[2 * m + cells],
+                           om[3 * m + 3 * cells], om[3 * m + 3 * cells + 1], om[3 * m + 3 * cells + 2]])
+        out["cells"][nc] = (cells, fields)
+        del om
+    return out
+
+
+def mesh_at(mesh: np.ndarray, nc: int, cells: np.ndarray) -> np.ndarray:
+    """The six fields (n, halo, c, vx, vy, vz) of a flat mesh at the given cells."""
+    m = nc ** 3
+    return np.stack([mesh[cells], mesh[m + cells], mesh[2 * m + cells],
+                     mesh[3 * m + 3 * cells], mesh[3 * m + 3 * cells + 1], mesh[3 * m + 3 * cells + 2]])

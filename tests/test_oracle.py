@@ -8,6 +8,7 @@ import pytest
 
 import asmooth_amd as A
 import oracle as O
+from tests import helpers
 from tests.helpers import mesh_fields, small_set
 
 needs_ref = pytest.mark.skipif(not O.have_ref(), reason="reference tree absent: oracle/_ref not built")
@@ -309,3 +310,27 @@ def test_node_shares_partition_the_local_particles():
         assert np.all((s["x"][:nl] >= 0) & (s["x"][:nl] < L / 2))
         tot += nl
     assert tot == len(xv)
+
+
+@pytest.mark.parametrize("centre", [(24.0, 40.0, 60.0), (1.0, 2.0, 94.0)])
+def test_region_oracle_equals_whole_set_oracle(centre):
+    # The sampled-region check used at full size gives, for its particles and
+    # cells, exactly what the oracle gives on the whole set.
+    b, nl, L, bf = _buf("zeldovich", 48, 5)
+    ncs = [16, 48]
+    pos, vel = b[:, :3], b[:, 3:6]
+    nbr, _, rk = O.knn(pos)
+    dens64 = O.density(pos, nbr, rk)
+    hmax = float(rk.max())
+    r = helpers.region_oracle(b, L, np.array(centre), half=8.0, hmax=hmax, ncs=ncs)
+    i = r["index"]
+    assert 0 < len(i) < len(b)
+    assert np.array_equal(r["rk"].view(np.int32), rk[i].view(np.int32))
+    assert np.array_equal(r["nbr"], nbr[i])
+    assert np.array_equal(r["dens64"], dens64[i])
+    dens32 = dens64.astype(np.float32)
+    for nc in ncs:
+        cells, o6 = r["cells"][nc]
+        assert len(cells) > 0
+        whole = O.assign_on_mesh(pos, vel, dens32, rk, np.zeros(len(b), np.int32), L, nc, -1)
+        assert np.array_equal(o6, helpers.mesh_at(whole, nc, cells))
