@@ -20,6 +20,13 @@ namespace {
 
 constexpr int kTileMax = 8;                 // cells per tile side (LDS sized for 8^3)
 constexpr int kTileCells = kTileMax * kTileMax * kTileMax;
+// Padded accumulator layout of the packed kernel: cell (x, y, z) of a tile
+// at x + kPadY * y + kPadZ * z.  With rows of 8 doubles the five or so rows
+// of one particle's box start on the same two of the 32 banks, so the lanes
+// of a sweep pile onto a few banks; an odd row pitch spreads them.
+constexpr int kPadY = kTileMax + 1;
+constexpr int kPadZ = kTileMax * kPadY;
+constexpr int kPadCells = kTileMax * kPadZ;
 constexpr int kDepThreads = 256;
 
 __device__ __forceinline__ void wave_lds_fence_dep()
@@ -339,10 +346,11 @@ __global__ void deposit_fill_kernel(DepositArgs a, const uint32_t* __restrict__ 
 }
 
 // Write one tile's accumulators to the mesh (FP32), every cell once.
-template <typename Acc>
-__device__ __forceinline__ void flush_tile(const DepositArgs& a, const Acc (*acc)[kTileCells], int cx0, int cy0,
+template <typename Acc, int kCells = kTileCells>
+__device__ __forceinline__ void flush_tile(const DepositArgs& a, const Acc (*acc)[kCells], int cx0, int cy0,
                                            int cz0, float* __restrict__ mesh, int accumulate)
 {
+    static_assert(kCells == kTileCells || kCells == kPadCells, "tile layout");
     const int tid = threadIdx.x;
     const int T = a.tile;
     const int ncell = T * T * T;
@@ -353,9 +361,10 @@ __device__ __forceinline__ void flush_tile(const DepositArgs& a, const Acc (*acc
         const int gx = cx0 + lx, gy = cy0 + ly, gz = cz0 + lz;
         if (gx >= a.nc || gy >= a.nc || gz >= a.nc) continue;
         const int64_t idx = (static_cast<int64_t>(gz) * nc + gy) * nc + gx;
+        const int ac = kCells == kPadCells ? lz * kPadZ + ly * kPadY + lx : c;
         float vals[6];
 #pragma unroll
-        for (int f = 0; f < 6; ++f) vals[f] = static_cast<float>(acc[f][c]);
+        for (int f = 0; f < 6; ++f) vals[f] = static_cast<float>(acc[f][ac]);
         if (accumulate) {
             mesh[idx] += vals[0];
             mesh[nmesh + idx] += vals[1];
@@ -478,9 +487,9 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
 // useful work whatever the box shapes, and each term is the reference's
 // FP32 product ((wx * wy) * wz, coarse_mesh.cpp:93).
 //
-// Eight wavefronts share a tile's accumulators: the tile's FP64 sums (24 KB)
-// plus 6.5 KB of weights and marks per wavefront let two workgroups, 16
-// wavefronts, share a CU's LDS.
+// Eight wavefronts share a tile's accumulators: the tile's padded FP64 sums
+// (27 KB) plus 6.5 KB of weights and marks per wavefront let two workgroups,
+// 16 wavefronts, share a CU's 160 KB of LDS.
 constexpr int kPackThreads = 512;
 constexpr int kDepWaves = kPackThreads / kWave;
 constexpr int kWStride = 3 * kTileMax + 1;   // odd row stride: entries fall in different banks
@@ -490,7 +499,7 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
     DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
     const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
 {
-    __shared__ Acc acc[6][kTileCells];
+    __shared__ Acc acc[6][kPadCells];
     __shared__ float wts[kDepWaves][kWave * kWStride];
     __shared__ int smap[kDepWaves][kWave];
     const int tid = threadIdx.x;
@@ -504,7 +513,7 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
     const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
     const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
     const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
-    for (int c = tid; c < 6 * kTileCells; c += kPackThreads) (&acc[0][0])[c] = Acc(0);
+    for (int c = tid; c < 6 * kPadCells; c += kPackThreads) (&acc[0][0])[c] = Acc(0);
     map[lane] = -1;
     __syncthreads();
 
@@ -588,7 +597,7 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
                 // coarse_mesh.cpp:93: ker3 = (wx * wy) * wz
                 float ker3 = wp[ix] * wp[kTileMax + iy];
                 ker3 = ker3 * wp[2 * kTileMax + iz];
-                const int c = ((bz + iz) * T + (by + iy)) * T + bx + ix;
+                const int c = (bz + iz) * kPadZ + (by + iy) * kPadY + bx + ix;
                 if (igm) {
                     atomicAdd(&acc[0][c], static_cast<Acc>(ker3));
                     atomicAdd(&acc[2][c], static_cast<Acc>(ker3 * p_dens));
