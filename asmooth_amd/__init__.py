@@ -48,6 +48,9 @@ OPT_TILE = 4
 OPT_DETERMINISTIC = 5
 OPT_KNN_METHOD = 6
 OPT_DEPOSIT_ACCUM = 7
+# fields of asm_stitch_mesh (FIELD_NTOT and FIELD_MOMENTUM as tools/reformat2.cpp derives them)
+FIELD_N, FIELD_HALO, FIELD_C, FIELD_VEL, FIELD_NTOT, FIELD_MOMENTUM = range(6)
+FIELD_COMPONENTS = (1, 1, 1, 3, 1, 3)
 
 
 class AsmoothError(RuntimeError):
@@ -88,6 +91,10 @@ def lib() -> C.CDLL:
         "asm_normalize_mesh": (i32, [vp, i32]),
         "asm_get_mesh": (i32, [vp, i32, vp]),
         "asm_mesh_device_ptr": (vp, [vp, i32]),
+        "asm_stitch_mesh": (i32, [vp, i32, i32, P(i32), i32, vp]),
+        "asm_reformat": (i32, [i32, C.c_char_p, C.c_char_p, i32, P(f64)]),
+        "asm_reformat2": (i32, [i32, C.c_char_p, C.c_char_p, C.c_char_p, P(f64)]),
+        "asm_device_to_host": (i32, [vp, vp, i64]),
         "asm_get_particles": (i32, [vp, vp]),
         "asm_get_rk_dens": (i32, [vp, vp, vp, vp]),
         "asm_get_neighbors": (i32, [vp, vp]),
@@ -259,6 +266,14 @@ class Context:
 
     def mesh_device_ptr(self, nc: int) -> int:
         return int(self._L.asm_mesh_device_ptr(self._h, int(nc)) or 0)
+
+    def stitch_mesh(self, nc: int, nn: int, coord, field: int, dst_ptr: int) -> None:
+        """Write this context's normalised mesh for nc, as node `coord` of an
+        nn^3 decomposition, into the global device array at dst_ptr (one of
+        the FIELD_* fields; runs on the context's stream)."""
+        c3 = (C.c_int * 3)(*[int(v) for v in coord])
+        self._check(self._L.asm_stitch_mesh(self._h, int(nc), int(nn), c3, int(field), C.c_void_p(dst_ptr)),
+                    "asm_stitch_mesh")
 
     def get_particles(self) -> np.ndarray:
         out = np.empty(self.n, dtype=PARTICLE_DTYPE)
@@ -596,3 +611,52 @@ def write_mesh_separate(base: str, inode: int, mesh: np.ndarray, nc: int) -> Non
     rc = lib().asm_write_mesh_separate(base.encode(), int(inode), _ptr(m), int(nc))
     if rc < 0:
         raise AsmoothError(f"asm_write_mesh_separate({base}) failed with status {rc}")
+
+
+def reformat(nn: int, out_dir: str, file_base: str, n_components: int) -> float:
+    """tools/reformat.cpp: stitch nn^3 node files <file_base><inode>.dat into
+    <out_dir>/<file_base>_all.dat; returns the total the tool prints."""
+    tot = C.c_double()
+    rc = lib().asm_reformat(int(nn), out_dir.encode(), file_base.encode(), int(n_components), C.byref(tot))
+    if rc < 0:
+        raise AsmoothError(f"asm_reformat({file_base}) failed with status {rc}")
+    return tot.value
+
+
+def reformat2(nn: int, out_dir: str, redshift: str, in_dir: str) -> tuple[float, float]:
+    """tools/reformat2.cpp: n, ntot and momentum global files from the node
+    files of write_mesh_separate; returns the two totals the tool prints."""
+    tot = (C.c_double * 2)()
+    rc = lib().asm_reformat2(int(nn), out_dir.encode(), redshift.encode(), in_dir.encode(), tot)
+    if rc < 0:
+        raise AsmoothError(f"asm_reformat2({redshift}) failed with status {rc}")
+    return tot[0], tot[1]
+
+
+class DeviceArray:
+    """Raw device allocation of n float32 (for asm_stitch_mesh destinations)."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        p = C.c_void_p()
+        if lib().asm_device_alloc(C.byref(p), 4 * self.n) != 0:
+            raise AsmoothError("device allocation failed")
+        self.ptr = p.value
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.n, np.float32)
+        if lib().asm_device_to_host(_ptr(out), C.c_void_p(self.ptr), 4 * self.n) != 0:
+            raise AsmoothError("device to host copy failed")
+        return out
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().asm_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+

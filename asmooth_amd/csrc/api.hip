@@ -118,7 +118,8 @@ struct asm_context {
     DevBuf<int32_t> nbr;
     DevBuf<KnnStats> kstats;   // kStatSlots copies
     // deposition
-    DevBuf<uint32_t> tile_cnt, tile_start, tile_cur, list, scan_total;
+    DevBuf<uint32_t> tile_cnt, tile_start, tile_cur, list;
+    DevBuf<uint64_t> scan_total;
     DevBuf<double> red_partial, red_out;
     std::map<int, MeshDev> meshes;
     // halos
@@ -556,7 +557,7 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     HIPC(ctx->tile_start.ensure(ntiles));
     HIPC(ctx->tile_cur.ensure(ntiles));
     HIPC(ctx->scan_total.ensure(1));
-    uint32_t total = 0;
+    uint64_t total = 0;
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cnt.p, ntiles, s));
@@ -565,8 +566,13 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
         HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, s));
     }
     if (!ctx->capturing) {
-        HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
+        // List offsets are 32-bit: more (particle, tile) entries than that
+        // would wrap them.
+        if (total > 0xffffffffull)
+            return fail(ctx, ASM_ERR_CAPACITY, "nc %d with tile %d needs %llu list entries, more than 2^32 - 1; use a larger tile",
+                        nc, a.tile, static_cast<unsigned long long>(total));
         HIPC(ctx->list.ensure(total > 0 ? total : 1));
     }
     a.list_cap = static_cast<uint32_t>(ctx->list.cap < 0xffffffffull ? ctx->list.cap : 0xffffffffull);
@@ -796,6 +802,22 @@ int asm_normalize_mesh(asm_context* ctx, int nc)
     const float vol = std::pow(ctx->boxsize / static_cast<float>(nc), 3.0f);
     StageTimer t(ctx, T_NORMALIZE);
     HIPC(launch_normalize(it->second.mesh.p, nc, vol, ctx->stream));
+    return ASM_OK;
+}
+
+int asm_stitch_mesh(asm_context* ctx, int nc, int nn, const int coord[3], int field, float* dst)
+{
+    if (!ctx) return ASM_ERR_ARG;
+    if (!coord || !dst) return fail(ctx, ASM_ERR_ARG, "null argument");
+    if (nn <= 0 || field < 0 || field > 5) return fail(ctx, ASM_ERR_ARG, "nn %d or field %d out of range", nn, field);
+    for (int a = 0; a < 3; ++a)
+        if (coord[a] < 0 || coord[a] >= nn) return fail(ctx, ASM_ERR_ARG, "node coordinate outside 0..%d", nn - 1);
+    if (static_cast<int64_t>(nc) * nn > 0x7fffffffLL)
+        return fail(ctx, ASM_ERR_CAPACITY, "global side %lld beyond int32", static_cast<long long>(nc) * nn);
+    auto it = ctx->meshes.find(nc);
+    if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(launch_stitch(it->second.mesh.p, nc, nn, coord, field, dst, ctx->stream));
     return ASM_OK;
 }
 

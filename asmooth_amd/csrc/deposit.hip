@@ -787,6 +787,43 @@ __global__ void normalize_kernel(float* __restrict__ mesh, int64_t nmesh, float 
     halo[j] = halo[j] * 8.0f;
 }
 
+// One node's normalised mesh placed into a global array of side G = nc * nn
+// at cell offset (ox, oy, oz), x fastest, as the stitching tools lay it out
+// (tools/reformat.cpp:81-107).  Fields: 0 n, 1 halo, 2 c, 3 vel (3
+// components interleaved, as in the node mesh), 4 ntot = h + n and 5 the
+// momentum vel * (n / 8) (tools/reformat2.cpp:118-126).  One thread per
+// output float, so consecutive threads write consecutive words of a row.
+__global__ void stitch_kernel(const float* __restrict__ mesh, int nc, int64_t G, int ox, int oy, int oz, int field,
+                              float* __restrict__ dst)
+{
+    const int comps = (field == 3 || field == 5) ? 3 : 1;
+    const int64_t nmesh = static_cast<int64_t>(nc) * nc * nc;
+    const int64_t total = nmesh * comps;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const int64_t cell = t / comps;
+        const int64_t a = t - cell * comps;
+        const int64_t lx = cell % nc;
+        const int64_t ly = (cell / nc) % nc;
+        const int64_t lz = cell / (static_cast<int64_t>(nc) * nc);
+        const int64_t g = ((oz + lz) * G + (oy + ly)) * G + (ox + lx);
+        float v;
+        if (field == 0)
+            v = mesh[cell];
+        else if (field == 1)
+            v = mesh[nmesh + cell];
+        else if (field == 2)
+            v = mesh[2 * nmesh + cell];
+        else if (field == 3)
+            v = mesh[3 * nmesh + t];
+        else if (field == 4)
+            v = mesh[nmesh + cell] + mesh[cell];
+        else
+            v = mesh[3 * nmesh + t] * (mesh[cell] / 8.0f);
+        dst[g * comps + a] = v;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_deposit_count(const DepositArgs& a, uint32_t* tile_cnt, hipStream_t s)
@@ -870,6 +907,16 @@ hipError_t launch_normalize(float* mesh, int nc, float vol, hipStream_t s)
 {
     const int64_t nmesh = static_cast<int64_t>(nc) * nc * nc;
     normalize_kernel<<<static_cast<unsigned>((nmesh + 255) / 256), 256, 0, s>>>(mesh, nmesh, vol);
+    return hipGetLastError();
+}
+
+hipError_t launch_stitch(const float* mesh, int nc, int nn, const int coord[3], int field, float* dst, hipStream_t s)
+{
+    const int64_t total = static_cast<int64_t>(nc) * nc * nc * ((field == 3 || field == 5) ? 3 : 1);
+    const int64_t want = (total + 255) / 256;
+    const unsigned blocks = static_cast<unsigned>(want < 65536 ? want : 65536);
+    stitch_kernel<<<blocks, 256, 0, s>>>(mesh, nc, static_cast<int64_t>(nc) * nn, coord[0] * nc, coord[1] * nc,
+                                         coord[2] * nc, field, dst);
     return hipGetLastError();
 }
 

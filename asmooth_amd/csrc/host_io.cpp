@@ -1,9 +1,11 @@
 // Host-side steps either side of the device path: the buffered particle set
-// of a node, the cubep3m xv reader/writer and the mesh file writer.
+// of a node, the cubep3m xv reader/writer, the mesh file writer and the
+// stitching of node mesh files into global ones.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/asmooth.h"
@@ -327,6 +329,170 @@ int asm_write_mesh_separate(const char* base, int inode, const float* mesh, int 
         if (!ok) return ASM_ERR_IO;
     }
     return ASM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Files of one z-slab of nodes, closed on scope exit.
+struct NodeFiles {
+    std::vector<FILE*> fp;
+    explicit NodeFiles(size_t n) : fp(n, nullptr) {}
+    ~NodeFiles()
+    {
+        for (FILE* f : fp)
+            if (f) std::fclose(f);
+    }
+};
+
+std::string node_name(const char* fmt, const char* a, const char* b, int inode)
+{
+    std::vector<char> buf(std::strlen(a) + (b ? std::strlen(b) : 0) + std::strlen(fmt) + 16);
+    if (b)
+        std::snprintf(buf.data(), buf.size(), fmt, a, b, inode);
+    else
+        std::snprintf(buf.data(), buf.size(), fmt, a, inode);
+    return std::string(buf.data());
+}
+
+// Opens one node file and reads its int32 cell count; every file of a
+// stitch must hold the same count.
+int open_node(const std::string& name, FILE** fp, int* nc)
+{
+    *fp = std::fopen(name.c_str(), "rb");
+    if (!*fp) return ASM_ERR_IO;
+    int32_t n = 0;
+    if (std::fread(&n, sizeof(int32_t), 1, *fp) != 1 || n <= 0) return ASM_ERR_IO;
+    if (*nc == 0) *nc = n;
+    return n == *nc ? ASM_OK : ASM_ERR_IO;
+}
+
+int write_header(FILE* fp, int64_t global)
+{
+    if (global > 0x7fffffff) return ASM_ERR_CAPACITY;
+    const int32_t h[3] = {static_cast<int32_t>(global), static_cast<int32_t>(global), static_cast<int32_t>(global)};
+    return std::fwrite(h, sizeof(int32_t), 3, fp) == 3 ? ASM_OK : ASM_ERR_IO;
+}
+
+}  // namespace
+
+extern "C" {
+
+// tools/reformat.cpp:11-123.  Node files <file_base><inode>.dat (inode =
+// ix + nn iy + nn^2 iz), each int32 nc then nc^3 cells of n_components
+// floats with x fastest, become <out_dir>/<file_base>_all.dat: three int32
+// nc*nn, then the global cells with x fastest.  One z-slab of nodes is open
+// at a time and rows of nc cells are copied in global order (:55-106).
+// *total is the sum of the first nc floats of every row copied, accumulated
+// in long double in that order (:104-105; for one component, the sum of the
+// field).  Node files longer than nc^3 cells are read only that far, as
+// there.
+int asm_reformat(int nn, const char* out_dir, const char* file_base, int n_components, double* total)
+{
+    if (nn <= 0 || !out_dir || !file_base || n_components <= 0) return ASM_ERR_ARG;
+    FILE* out = std::fopen(node_name("%s/%s_all.dat", out_dir, file_base, 0).c_str(), "wb");
+    if (!out) return ASM_ERR_IO;
+    long double sum = 0.0L;
+    int nc = 0;
+    int rc = ASM_OK;
+    std::vector<float> row;
+    for (int iz_node = 0; iz_node < nn && rc == ASM_OK; ++iz_node) {
+        NodeFiles f(static_cast<size_t>(nn) * nn);
+        for (int k = 0; k < nn * nn && rc == ASM_OK; ++k)
+            rc = open_node(node_name("%s%d.dat", file_base, nullptr, nn * nn * iz_node + k), &f.fp[k], &nc);
+        if (rc != ASM_OK) break;
+        const size_t ndata = static_cast<size_t>(nc) * n_components;
+        row.resize(ndata);
+        if (iz_node == 0) rc = write_header(out, static_cast<int64_t>(nc) * nn);
+        for (int iz = 0; iz < nc && rc == ASM_OK; ++iz)
+            for (int iy_node = 0; iy_node < nn && rc == ASM_OK; ++iy_node)
+                for (int iy = 0; iy < nc && rc == ASM_OK; ++iy)
+                    for (int ix_node = 0; ix_node < nn; ++ix_node) {
+                        FILE* in = f.fp[iy_node * nn + ix_node];
+                        if (std::fread(row.data(), sizeof(float), ndata, in) != ndata ||
+                            std::fwrite(row.data(), sizeof(float), ndata, out) != ndata) {
+                            rc = ASM_ERR_IO;
+                            break;
+                        }
+                        for (int i = 0; i < nc; ++i) sum += row[i];
+                    }
+    }
+    if (std::fclose(out) != 0 && rc == ASM_OK) rc = ASM_ERR_IO;
+    if (total) *total = static_cast<double>(sum);
+    return rc;
+}
+
+// tools/reformat2.cpp:10-171.  From the n, h and vel node files of
+// write_mesh_separate, <in_dir>/<redshift>{n,h,vel}<inode>.dat, writes
+// <out_dir>/<redshift>n_all.dat (n as stored), ntot_all.dat (h + n) and
+// v_all.dat (vel * (n / 8), the momentum), stitched as asm_reformat does.
+// totals[0] = sum of n / 8 and totals[1] = sum of (h + n) / 8, in long
+// double in file order (:118-126).
+int asm_reformat2(int nn, const char* out_dir, const char* redshift, const char* in_dir, double totals[2])
+{
+    if (nn <= 0 || !out_dir || !redshift || !in_dir) return ASM_ERR_ARG;
+    const char* tags_out[3] = {"%s/%sn_all.dat", "%s/%sntot_all.dat", "%s/%sv_all.dat"};
+    const char* tags_in[3] = {"/%sn%d.dat", "/%sh%d.dat", "/%svel%d.dat"};
+    FILE* out[3] = {nullptr, nullptr, nullptr};
+    int rc = ASM_OK;
+    for (int f = 0; f < 3 && rc == ASM_OK; ++f) {
+        out[f] = std::fopen(node_name(tags_out[f], out_dir, redshift, 0).c_str(), "wb");
+        if (!out[f]) rc = ASM_ERR_IO;
+    }
+    long double t1 = 0.0L, t2 = 0.0L;
+    int nc = 0;
+    std::vector<float> bn, bh, bv;
+    for (int iz_node = 0; iz_node < nn && rc == ASM_OK; ++iz_node) {
+        NodeFiles f(3 * static_cast<size_t>(nn) * nn);
+        for (int k = 0; k < nn * nn && rc == ASM_OK; ++k)
+            for (int g = 0; g < 3 && rc == ASM_OK; ++g) {
+                const std::string fmt = std::string("%s") + tags_in[g];
+                rc = open_node(node_name(fmt.c_str(), in_dir, redshift, nn * nn * iz_node + k), &f.fp[3 * k + g],
+                               &nc);
+            }
+        if (rc != ASM_OK) break;
+        const size_t m = static_cast<size_t>(nc);
+        bn.resize(m);
+        bh.resize(m);
+        bv.resize(3 * m);
+        if (iz_node == 0)
+            for (int g = 0; g < 3 && rc == ASM_OK; ++g) rc = write_header(out[g], static_cast<int64_t>(nc) * nn);
+        for (int iz = 0; iz < nc && rc == ASM_OK; ++iz)
+            for (int iy_node = 0; iy_node < nn && rc == ASM_OK; ++iy_node)
+                for (int iy = 0; iy < nc && rc == ASM_OK; ++iy)
+                    for (int ix_node = 0; ix_node < nn; ++ix_node) {
+                        FILE** in = &f.fp[3 * (iy_node * nn + ix_node)];
+                        if (std::fread(bn.data(), sizeof(float), m, in[0]) != m ||
+                            std::fread(bh.data(), sizeof(float), m, in[1]) != m ||
+                            std::fread(bv.data(), sizeof(float), 3 * m, in[2]) != 3 * m) {
+                            rc = ASM_ERR_IO;
+                            break;
+                        }
+                        for (size_t j = 0; j < m; ++j) {
+                            const float n = bn[j] / 8.0f;
+                            t1 += n;
+                            bh[j] += bn[j];
+                            t2 += bh[j] / 8.0f;
+                            bv[3 * j] *= n;
+                            bv[3 * j + 1] *= n;
+                            bv[3 * j + 2] *= n;
+                        }
+                        if (std::fwrite(bn.data(), sizeof(float), m, out[0]) != m ||
+                            std::fwrite(bh.data(), sizeof(float), m, out[1]) != m ||
+                            std::fwrite(bv.data(), sizeof(float), 3 * m, out[2]) != 3 * m) {
+                            rc = ASM_ERR_IO;
+                            break;
+                        }
+                    }
+    }
+    for (int f = 0; f < 3; ++f)
+        if (out[f] && std::fclose(out[f]) != 0 && rc == ASM_OK) rc = ASM_ERR_IO;
+    if (totals) {
+        totals[0] = static_cast<double>(t1);
+        totals[1] = static_cast<double>(t2);
+    }
+    return rc;
 }
 
 }  // extern "C"

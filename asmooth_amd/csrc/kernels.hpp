@@ -9,7 +9,9 @@
 namespace asm_hip {
 
 // ---- sort.hip --------------------------------------------------------------
-hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s);
+// In-place exclusive scan of m 32-bit counts; *total (if given) receives
+// their sum in 64 bits, so a caller can tell when the 32-bit offsets wrapped.
+hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint64_t* total, hipStream_t s);
 int64_t radix_hist_entries(int64_t n);
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
                              int64_t n, int bits, uint32_t* hist, hipStream_t s, bool* result_in_tmp);
@@ -154,6 +156,9 @@ hipError_t launch_fill_zero_f64(double* p, int64_t n, hipStream_t s);
 // out[3] = sum c (raw).  partial must hold 4 * 1024 doubles.
 hipError_t launch_mesh_reduce(const float* mesh, int nc, double* partial, double* out, hipStream_t s);
 hipError_t launch_normalize(float* mesh, int nc, float vol, hipStream_t s);
+// A node's normalised mesh (node coord of nn^3) into a global array of side
+// nc * nn: field 0 n, 1 halo, 2 c, 3 vel, 4 h + n, 5 vel * (n / 8).
+hipError_t launch_stitch(const float* mesh, int nc, int nn, const int coord[3], int field, float* dst, hipStream_t s);
 
 // ---- halo.hip ----------------------------------------------------------------
 struct HaloRec {

@@ -165,14 +165,19 @@ constexpr int kScanThreads = 1024;
 // In-place exclusive scan of m uint32 values by one workgroup; writes the
 // grand total to *total (if non-null).
 __global__ __launch_bounds__(kScanThreads) void scan_exclusive_1wg(uint32_t* __restrict__ a, int64_t m,
-                                                                   uint32_t* __restrict__ total)
+                                                                   uint64_t* __restrict__ total)
 {
     __shared__ uint32_t wsum[kScanThreads / kWave];
+    __shared__ uint64_t wsum64[kScanThreads / kWave];
     __shared__ uint32_t carry_s;
+    __shared__ uint64_t carry64_s;   // the same running total without wrap-around
     const int t = threadIdx.x;
     const int lane = t % kWave;
     const int w = t / kWave;
-    if (t == 0) carry_s = 0;
+    if (t == 0) {
+        carry_s = 0;
+        carry64_s = 0;
+    }
     __syncthreads();
     constexpr int kPer = 4;
     for (int64_t chunk = 0; chunk < m; chunk += static_cast<int64_t>(kScanThreads) * kPer) {
@@ -184,12 +189,19 @@ __global__ __launch_bounds__(kScanThreads) void scan_exclusive_1wg(uint32_t* __r
             x[i] = (b + i < m) ? a[b + i] : 0u;
             sum += x[i];
         }
+        uint64_t sum64 = 0;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) sum64 += x[i];
+        for (int o = kWave / 2; o > 0; o >>= 1) sum64 += __shfl_xor(sum64, o);
         uint32_t inc = sum;
         for (int o = 1; o < kWave; o <<= 1) {
             uint32_t y = __shfl_up(inc, o);
             if (lane >= o) inc += y;
         }
-        if (lane == kWave - 1) wsum[w] = inc;
+        if (lane == kWave - 1) {
+            wsum[w] = inc;
+            wsum64[w] = sum64;
+        }
         __syncthreads();
         uint32_t pre = 0, tot = 0;
         for (int ww = 0; ww < kScanThreads / kWave; ++ww) {
@@ -204,15 +216,20 @@ __global__ __launch_bounds__(kScanThreads) void scan_exclusive_1wg(uint32_t* __r
             run += x[i];
         }
         __syncthreads();
-        if (t == 0) carry_s = carry + tot;
+        if (t == 0) {
+            carry_s = carry + tot;
+            uint64_t tot64 = 0;
+            for (int ww = 0; ww < kScanThreads / kWave; ++ww) tot64 += wsum64[ww];
+            carry64_s += tot64;
+        }
         __syncthreads();
     }
-    if (t == 0 && total) *total = carry_s;
+    if (t == 0 && total) *total = carry64_s;
 }
 
 }  // namespace
 
-hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s)
+hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint64_t* total, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
     scan_exclusive_1wg<<<1, kScanThreads, 0, s>>>(a, m, total);

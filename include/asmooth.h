@@ -133,6 +133,23 @@ int asm_normalize_mesh(asm_context* ctx, int nc);
 int asm_get_mesh(asm_context* ctx, int nc, float* mesh);
 /* Device address of the mesh for nc (valid until the context is destroyed). */
 const float* asm_mesh_device_ptr(asm_context* ctx, int nc);
+/* The stitching of node meshes into one global mesh (tools/reformat.cpp:
+ * 55-106, tools/reformat2.cpp:82-142, which do it offline from the node files)
+ * done on the device: this context's mesh for nc, already normalised
+ * (asm_normalize_mesh), as node `coord` of an nn^3 decomposition, is written
+ * into dst, a device array of (nc*nn)^3 cells with x fastest, of one field:
+ *   ASM_FIELD_N, _HALO, _C (1 float per cell), _VEL (3), and the two fields
+ *   reformat2 derives: _NTOT = h + n (1) and _MOMENTUM = vel * (n / 8) (3).
+ * Runs on the context's stream (asm_synchronize before reading dst). */
+enum {
+    ASM_FIELD_N = 0,
+    ASM_FIELD_HALO = 1,
+    ASM_FIELD_C = 2,
+    ASM_FIELD_VEL = 3,
+    ASM_FIELD_NTOT = 4,
+    ASM_FIELD_MOMENTUM = 5
+};
+int asm_stitch_mesh(asm_context* ctx, int nc, int nn, const int coord[3], int field, float* dst);
 
 /* Results per particle in the caller's order: rk, dens and igrp are written,
  * x and v are copied back unchanged. */
@@ -187,6 +204,17 @@ void asm_free(void* p);
  * that is already normalised: <base>n<inode>.dat, h, c (int32 nc + nc^3
  * floats) and vel (int32 nc + 3 nc^3 floats). */
 int asm_write_mesh_separate(const char* base, int inode, const float* mesh, int nc);
+/* The offline stitching tools, with status codes instead of aborts.
+ * asm_reformat (tools/reformat.cpp:11-123): node files <file_base><inode>.dat
+ * (inode = ix + nn iy + nn^2 iz; int32 nc, then nc^3 cells of n_components
+ * floats) into <out_dir>/<file_base>_all.dat (3 x int32 nc*nn, then the
+ * global cells, x fastest); *total = sum of the first nc floats of each row,
+ * as the tool prints it.  asm_reformat2 (tools/reformat2.cpp:10-171): the
+ * n, h and vel files <in_dir>/<redshift>{n,h,vel}<inode>.dat into
+ * <out_dir>/<redshift>{n,ntot,v}_all.dat holding n, h + n and vel * (n / 8);
+ * totals = {sum n / 8, sum (h + n) / 8}. */
+int asm_reformat(int nn, const char* out_dir, const char* file_base, int n_components, double* total);
+int asm_reformat2(int nn, const char* out_dir, const char* redshift, const char* in_dir, double totals[2]);
 
 /* Autotuner (no counterpart in the reference; the reference has fixed
  * choices: leaf quota 16, kdtree_balanced.cpp:245, and no tiling).  After a
@@ -280,6 +308,8 @@ int asm_get_search_stats(asm_context* ctx, uint64_t out[4]);
 int asm_calibration_copy(void* dst, const void* src, int64_t bytes);
 int asm_device_alloc(void** p, int64_t bytes);
 int asm_device_free(void* p);
+/* Synchronous copy of `bytes` from device memory to host memory. */
+int asm_device_to_host(void* host, const void* dev, int64_t bytes);
 
 #ifdef __cplusplus
 }

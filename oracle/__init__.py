@@ -283,3 +283,34 @@ def ref_time_path(pos, vel, np_local: int, L: float, bf: float, ncs, nthreads: i
     rlib().ref_time_path(_p(pos), _p(vel), len(pos), int(np_local), float(L), float(bf), _p(nc_arr), len(nc_arr),
                          int(nthreads), _p(sec), _p(tot))
     return sec, tot
+
+
+# ------------------------------------------------------------- stitching ---
+
+FIELD_COMPONENTS = (1, 1, 1, 3, 1, 3)   # n, halo, c, vel, h + n, vel * (n / 8)
+
+
+def stitch_field(meshes: dict, nc: int, nn: int, field: int) -> np.ndarray:
+    """One field of the global mesh stitched from nn^3 normalised node meshes
+    (dict inode -> 6 nc^3 floats, inode = ix + nn iy + nn^2 iz), x fastest
+    over nc*nn cells per axis (tools/reformat.cpp:81-107).  Fields 4 and 5
+    are tools/reformat2.cpp:118-126 in float32: h + n and vel * (n / 8)."""
+    comps = FIELD_COMPONENTS[field]
+    G = nc * nn
+    m = nc ** 3
+    out = np.zeros((G, G, G, comps), np.float32)
+    for inode, mesh in meshes.items():
+        mesh = np.asarray(mesh, np.float32)
+        ix, iy, iz = inode % nn, (inode // nn) % nn, inode // (nn * nn)
+        n, h, c = mesh[:m], mesh[m:2 * m], mesh[2 * m:3 * m]
+        v = mesh[3 * m:6 * m].reshape(m, 3)
+        f = [n[:, None], h[:, None], c[:, None], v, (h + n)[:, None], v * (n / np.float32(8.0))[:, None]][field]
+        out[iz * nc:(iz + 1) * nc, iy * nc:(iy + 1) * nc, ix * nc:(ix + 1) * nc] = f.reshape(nc, nc, nc, comps)
+    return out.ravel()
+
+
+def ref_tool(name: str) -> str:
+    """Path of a stitching tool compiled from the reference (_ref/reformat,
+    _ref/reformat2), or '' when the reference tree was absent at build time."""
+    p = os.path.join(HERE, "_ref", name)
+    return p if os.path.exists(p) else ""

@@ -80,3 +80,126 @@ def test_halo_images_follow_the_particle_images():
     k = np.round(d / L)
     assert np.all(np.isin(k, [-1, 0, 1]))
     assert np.allclose(d - k * L, 0, atol=1e-4)
+
+
+# --------------------------------------------------- stitching (tools/) ---
+
+import subprocess  # noqa: E402
+
+import oracle as O  # noqa: E402
+
+
+def _node_meshes(nn, nc, seed):
+    """Normalised-looking node meshes: n > 0 or 0, halo, c, v (6 nc^3 floats)."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    m = nc ** 3
+    for inode in range(nn ** 3):
+        mesh = rng.standard_normal(6 * m).astype(np.float32)
+        mesh[:m] = np.abs(mesh[:m]) * 8.0
+        mesh[:m][rng.random(m) < 0.1] = 0.0
+        out[inode] = mesh
+    return out
+
+
+def test_stitch_oracle_places_every_cell():
+    # value = global cell index: the stitched field must be arange, whatever
+    # node and cell it came from (catches a swapped axis or node order)
+    nn, nc = 3, 4
+    G, m = nn * nc, nc ** 3
+    meshes = {}
+    for inode in range(nn ** 3):
+        ix, iy, iz = inode % nn, (inode // nn) % nn, inode // nn ** 2
+        lz, ly, lx = np.meshgrid(np.arange(nc), np.arange(nc), np.arange(nc), indexing="ij")
+        g = (((iz * nc + lz) * G + iy * nc + ly) * G + ix * nc + lx).ravel().astype(np.float32)
+        mesh = np.zeros(6 * m, np.float32)
+        mesh[:m], mesh[m:2 * m], mesh[2 * m:3 * m] = g, 2 * g, 3 * g
+        mesh[3 * m:] = np.stack([10 * g, 10 * g + 1, 10 * g + 2], 1).ravel()
+        meshes[inode] = mesh
+    a = np.arange(G ** 3, dtype=np.float32)
+    assert np.array_equal(O.stitch_field(meshes, nc, nn, 0), a)
+    assert np.array_equal(O.stitch_field(meshes, nc, nn, 1), 2 * a)
+    assert np.array_equal(O.stitch_field(meshes, nc, nn, 2), 3 * a)
+    assert np.array_equal(O.stitch_field(meshes, nc, nn, 3), np.stack([10 * a, 10 * a + 1, 10 * a + 2], 1).ravel())
+    assert np.array_equal(O.stitch_field(meshes, nc, nn, 4), 3 * a)
+    mom = np.stack([10 * a, 10 * a + 1, 10 * a + 2], 1) * (a / np.float32(8))[:, None]
+    assert np.array_equal(O.stitch_field(meshes, nc, nn, 5), mom.ravel())
+
+
+def _read_all(path):
+    raw = np.fromfile(path, np.int32, count=3)
+    return raw, np.fromfile(path, np.float32, offset=12)
+
+
+def test_reformat_matches_oracle_and_reference_tool(tmp_path, monkeypatch):
+    nn, nc, z = 2, 5, "1.500"
+    meshes = _node_meshes(nn, nc, 3)
+    (tmp_path / "in").mkdir()
+    (tmp_path / "out").mkdir()
+    (tmp_path / "ref").mkdir()
+    monkeypatch.chdir(tmp_path / "in")
+    for inode, mesh in meshes.items():
+        A.write_mesh_separate(z, inode, mesh, nc)
+    G = nn * nc
+    tool = O.ref_tool("reformat")
+    for tag, field in (("n", 0), ("h", 1), ("c", 2), ("vel", 3)):
+        comps = O.FIELD_COMPONENTS[field]
+        total = A.reformat(nn, "../out", z + tag, comps)
+        hdr, data = _read_all(tmp_path / "out" / f"{z}{tag}_all.dat")
+        assert list(hdr) == [G, G, G]
+        want = O.stitch_field(meshes, nc, nn, field)
+        assert np.array_equal(data, want)
+        # the tool sums the first nc floats of each row of nc * comps floats
+        rows = want.reshape(-1, nc * comps)[:, :nc].astype(np.float64)
+        assert total == pytest.approx(float(rows.sum()), rel=1e-12, abs=1e-9)
+        if tool:
+            r = subprocess.run([tool, str(nn), "../ref", z + tag, str(comps)], capture_output=True, text=True,
+                               timeout=60)
+            assert r.returncode == 0, r.stderr
+            ours = (tmp_path / "out" / f"{z}{tag}_all.dat").read_bytes()
+            assert ours == (tmp_path / "ref" / f"{z}{tag}_all.dat").read_bytes()
+            printed = float(r.stdout.split("total")[-1].split()[0])
+            assert total == pytest.approx(printed, rel=1e-5, abs=1e-5)
+
+
+def test_reformat2_matches_oracle_and_reference_tool(tmp_path, monkeypatch):
+    nn, nc, z = 2, 4, "0.000"
+    meshes = _node_meshes(nn, nc, 4)
+    for d in ("in", "out", "ref"):
+        (tmp_path / d).mkdir()
+    monkeypatch.chdir(tmp_path)
+    for inode, mesh in meshes.items():
+        A.write_mesh_separate(f"in/{z}", inode, mesh, nc)
+    t1, t2 = A.reformat2(nn, "out", z, "in")
+    G = nn * nc
+    for name, field in (("n", 0), ("ntot", 4), ("v", 5)):
+        hdr, data = _read_all(tmp_path / "out" / f"{z}{name}_all.dat")
+        assert list(hdr) == [G, G, G]
+        assert np.array_equal(data, O.stitch_field(meshes, nc, nn, field))
+    n = O.stitch_field(meshes, nc, nn, 0)
+    ntot = O.stitch_field(meshes, nc, nn, 4)
+    assert t1 == pytest.approx(float((n / np.float32(8)).astype(np.float64).sum()), rel=1e-12, abs=1e-9)
+    assert t2 == pytest.approx(float((ntot / np.float32(8)).astype(np.float64).sum()), rel=1e-12, abs=1e-9)
+    tool = O.ref_tool("reformat2")
+    if tool:
+        r = subprocess.run([tool, str(nn), "ref", z, "in"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stderr
+        for name in ("n", "ntot", "v"):
+            assert (tmp_path / "out" / f"{z}{name}_all.dat").read_bytes() == \
+                (tmp_path / "ref" / f"{z}{name}_all.dat").read_bytes()
+        words = r.stdout.split("total")[-1].split()
+        assert t1 == pytest.approx(float(words[1]), abs=1e-4)
+        assert t2 == pytest.approx(float(words[2]), abs=1e-4)
+
+
+def test_reformat_missing_node_file_is_an_error(tmp_path, monkeypatch):
+    meshes = _node_meshes(2, 3, 5)
+    (tmp_path / "out").mkdir()
+    monkeypatch.chdir(tmp_path)
+    for inode, mesh in meshes.items():
+        if inode != 6:
+            A.write_mesh_separate("z", inode, mesh, 3)
+    with pytest.raises(A.AsmoothError):
+        A.reformat(2, "out", "zc", 1)
+    with pytest.raises(A.AsmoothError):
+        A.reformat2(2, "out", "z", ".")
