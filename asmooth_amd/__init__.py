@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
         "asm_read_halo_file": (i32, [C.c_char_p, f32, vp, f32, f32, i32, i32, P(vp), P(i64)]),
         "asm_halo_delta_vir": (f64, [f64, f64]),
         "asm_read_xv": (i32, [C.c_char_p, P(vp), P(i64), vp]),
+        "asm_read_zip": (i32, [C.c_char_p, C.c_char_p, i32, i32, i32, vp, P(vp), P(i64), vp]),
         "asm_write_xv": (i32, [C.c_char_p, vp, i64, vp]),
         "asm_free": (None, [vp]),
         "asm_write_mesh_separate": (i32, [C.c_char_p, i32, vp, i32]),
@@ -596,6 +597,25 @@ def read_xv(path: str) -> tuple[np.ndarray, np.ndarray]:
     finally:
         L.asm_free(buf)
     return arr.reshape(-1, 6), hdr
+
+
+def read_zip(filebase: str, redshift: str, inode: int, nc_node_dim: int, mesh_scale: int = 4,
+             shift=(0.0, 0.0, 0.0)) -> tuple[np.ndarray, np.ndarray]:
+    """cubep3m zip reader (file_reader2.cpp:90-272): (xv (np, 6) float32,
+    zip0 header as 16 float32 words)."""
+    p, n = C.c_void_p(), C.c_int64()
+    hdr = np.zeros(16, np.float32)
+    sh = np.ascontiguousarray(shift, np.float32)
+    rc = lib().asm_read_zip(filebase.encode(), redshift.encode(), int(inode), int(nc_node_dim), int(mesh_scale),
+                            _ptr(sh), C.byref(p), C.byref(n), _ptr(hdr))
+    if rc < 0:
+        raise AsmoothError(f"asm_read_zip({filebase}{inode}/{redshift}) failed with status {rc}")
+    try:
+        xv = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n.value * 6,)).copy() \
+            if n.value else np.zeros(0, np.float32)
+    finally:
+        lib().asm_free(p)
+    return xv.reshape(-1, 6), hdr
 
 
 def write_xv(path: str, xv: np.ndarray, header=None) -> None:

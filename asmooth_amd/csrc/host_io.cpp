@@ -231,6 +231,110 @@ int asm_read_xv(const char* path, float** xv, int64_t* np, float header[11])
     return ASM_OK;
 }
 
+// cubep3m zip reader (file_reader2.cpp:90-272).  Four files per node,
+// <filebase><inode>/<redshift>zip{0,1,2,3}_<inode>.dat:
+//   zip0: 16-word header (int32 np_local, ..., float v_r2i at word 12), then
+//         3 bytes per particle, its offset in its coarse cell in 1/256ths;
+//   zip1: 16-word header, then 3 int16 velocities per particle;
+//   zip2: one byte per coarse cell (i fastest over nc_node_dim^3 cells), the
+//         particle count, 255 meaning "read it from zip3";
+//   zip3: int32 counts of the cells marked 255, in cell order.
+// Position x = mesh_scale * ((b + 0.5) / 256 + i - 1) - shift and velocity
+// v = s / v_r2i, in FP32 and in that order (:226-235).  zip0 and zip1 must end
+// exactly after the last particle and the counts must add up to np_local
+// (:240-256).  No byte swapping: little-endian files, as the default build.
+int asm_read_zip(const char* filebase, const char* redshift, int inode, int nc_node_dim, int mesh_scale,
+                 const float shift[3], float** xv, int64_t* np, float header[16])
+{
+    if (!filebase || !redshift || !xv || !np || nc_node_dim <= 0 || mesh_scale <= 0) return ASM_ERR_ARG;
+    FILE* fp[4] = {nullptr, nullptr, nullptr, nullptr};
+    auto close_all = [&]() {
+        for (FILE*& f : fp)
+            if (f) {
+                std::fclose(f);
+                f = nullptr;
+            }
+    };
+    for (int z = 0; z < 4; ++z) {
+        std::vector<char> name(std::strlen(filebase) + std::strlen(redshift) + 64);
+        std::snprintf(name.data(), name.size(), "%s%d/%szip%d_%d.dat", filebase, inode, redshift, z, inode);
+        fp[z] = std::fopen(name.data(), "rb");
+        if (!fp[z]) {
+            close_all();
+            return ASM_ERR_IO;
+        }
+    }
+    int32_t h0[16];
+    float h1[16];
+    if (std::fread(h0, sizeof(int32_t), 16, fp[0]) != 16 || std::fread(h1, sizeof(float), 16, fp[1]) != 16 ||
+        h0[0] < 0) {
+        close_all();
+        return ASM_ERR_IO;
+    }
+    const int64_t n = h0[0];
+    float v_r2i;
+    std::memcpy(&v_r2i, &h0[12], sizeof(float));
+    const float sh[3] = {shift ? shift[0] : 0.0f, shift ? shift[1] : 0.0f, shift ? shift[2] : 0.0f};
+    float* out = static_cast<float*>(std::malloc(sizeof(float) * 6 * static_cast<size_t>(n > 0 ? n : 1)));
+    if (!out) {
+        close_all();
+        return ASM_ERR_CAPACITY;
+    }
+    const float ms = static_cast<float>(mesh_scale);
+    int64_t k_out = 0;
+    int rc = ASM_OK;
+    const int nd = nc_node_dim;
+    for (int k = 1; k <= nd && rc == ASM_OK; ++k)
+        for (int j = 1; j <= nd && rc == ASM_OK; ++j)
+            for (int i = 1; i <= nd; ++i) {
+                unsigned char c8;
+                if (std::fread(&c8, 1, 1, fp[2]) != 1) {
+                    rc = ASM_ERR_IO;
+                    break;
+                }
+                int32_t cnt = c8;
+                if (cnt == 255 && std::fread(&cnt, sizeof(int32_t), 1, fp[3]) != 1) {
+                    rc = ASM_ERR_IO;
+                    break;
+                }
+                if (cnt < 0 || k_out + cnt > n) {
+                    rc = ASM_ERR_IO;
+                    break;
+                }
+                const float cell[3] = {static_cast<float>(i), static_cast<float>(j), static_cast<float>(k)};
+                for (int32_t l = 0; l < cnt; ++l) {
+                    unsigned char b[3];
+                    int16_t v[3];
+                    if (std::fread(b, 1, 3, fp[0]) != 3 || std::fread(v, sizeof(int16_t), 3, fp[1]) != 3) {
+                        rc = ASM_ERR_IO;
+                        break;
+                    }
+                    float* o = out + 6 * k_out;
+                    for (int a = 0; a < 3; ++a) {
+                        const float frac = (static_cast<float>(b[a]) + 0.5f) / 256.0f;
+                        o[a] = ms * ((frac + cell[a]) - 1.0f) - sh[a];
+                        o[3 + a] = static_cast<float>(v[a]) / v_r2i;
+                    }
+                    ++k_out;
+                }
+                if (rc != ASM_OK) break;
+            }
+    if (rc == ASM_OK) {
+        unsigned char extra;
+        if (k_out != n || std::fread(&extra, 1, 1, fp[0]) != 0 || std::fread(&extra, 1, 1, fp[1]) != 0)
+            rc = ASM_ERR_IO;
+    }
+    close_all();
+    if (rc != ASM_OK) {
+        std::free(out);
+        return rc;
+    }
+    *xv = out;
+    *np = n;
+    if (header) std::memcpy(header, h0, sizeof(h0));
+    return ASM_OK;
+}
+
 int asm_write_xv(const char* path, const float* xv, int64_t np, const float header[11])
 {
     if (!path || !xv || np < 0 || np > 0x7fffffff) return ASM_ERR_ARG;

@@ -1,16 +1,19 @@
 // clumping_tree_gpu: host driver of the device smoothing path with the
 // reference driver's command line and output files (clumping_tree_main.cpp).
 //
-//   clumping_tree_gpu -xv <L> -nc 32,64 [-buffer_factor 0.05]
-//                     [-node_dir results/node] [-pm_redshift redshifts.txt]
-//                     [-allocate MB] [-device 0] [-halo_excise 0|1]
+//   clumping_tree_gpu (-xv <L> | -nc_node_dim <n> [-mesh_scale 4]) -nc 32,64
+//                     [-buffer_factor 0.05] [-node_dir results/node]
+//                     [-pm_redshift redshifts.txt] [-allocate MB] [-device 0]
+//                     [-halo_excise 0|1]
 //
 // One process handles one node share on one GPU (node index 0, single
 // periodic rank: the 26 neighbours are the node itself, as with `mpiexec -n 1`
-// of the reference).  Per redshift it reads <node_dir>0/<z>xv0.dat
-// (file_reader2.cpp:24-86), subtracts shift/<z>shift.txt if present
-// (clumping_tree_main.cpp:323-347), builds the buffered set, runs the device
-// path and writes so/nc<nc>/<z>{n,h,c,vel}0.dat (coarse_mesh.cpp:173-201).
+// of the reference).  Per redshift it reads shift/<z>shift.txt if present
+// (clumping_tree_main.cpp:323-347), then <node_dir>0/<z>xv0.dat
+// (file_reader2.cpp:24-86) less the shift, or with -nc_node_dim the zip
+// files <node_dir>0/<z>zip{0,1,2,3}_0.dat (file_reader2.cpp:90-272), builds
+// the buffered set, runs the device path and writes
+// so/nc<nc>/<z>{n,h,c,vel}0.dat (coarse_mesh.cpp:173-201).
 #include <sys/stat.h>
 #include <sys/time.h>
 
@@ -94,7 +97,9 @@ int main(int argc, char** argv)
                     "\t-node_dir <node directory>; -node_dir results/node\n"
                     "\t-allocate <MB for particles> (accepted; device buffers size themselves)\n"
                     "\t-buffer_factor 0.05\n"
-                    "\t-xv <local boxsize>\n"
+                    "\t-xv <local boxsize> (xv particle format)\n"
+                    "\t-nc_node_dim 32 (for zip particle format)\n"
+                    "\t-mesh_scale 4   (for zip particle format)\n"
                     "\t-nc <nc1,nc2,...> list of number of mesh\n"
                     "\t-device <hip device ordinal>\n"
                     "\t-halo_excise 1 (excise SO halos of <node_dir>0/<z>halo0.dat)\n"
@@ -104,10 +109,16 @@ int main(int argc, char** argv)
         return 0;
     }
     auto op = parse_options(argc, argv);
-    const float boxsize = static_cast<float>(std::atof(op["-xv"].c_str()));
+    // -xv <L>: xv files of cube side L; otherwise the zip format, whose cube
+    // is mesh_scale * nc_node_dim (clumping_tree_main.cpp:103-127, file_reader2.cpp:268).
+    const int nc_node_dim = std::atoi(op["-nc_node_dim"].c_str());
+    const int mesh_scale = std::atoi(op["-mesh_scale"].c_str());
+    const bool read_xv_file = std::atof(op["-xv"].c_str()) > 0.0;
+    const float boxsize = read_xv_file ? static_cast<float>(std::atof(op["-xv"].c_str()))
+                                       : static_cast<float>(mesh_scale) * static_cast<float>(nc_node_dim);
     const float bf = static_cast<float>(std::atof(op["-buffer_factor"].c_str()));
     if (!(boxsize > 0.0f)) {
-        std::fprintf(stderr, "Error: only the xv particle format is supported; give -xv <boxsize>\n");
+        std::fprintf(stderr, "Error: option -nc_node_dim not given (or give -xv <boxsize>)\n");
         return 1;
     }
     if (!(bf > 0.0f)) {
@@ -150,17 +161,6 @@ int main(int argc, char** argv)
     for (const std::string& z : redshifts) {
         std::printf("z %s\n", z.c_str());
         double t_io0 = now_s();
-        char fname[512];
-        std::snprintf(fname, sizeof(fname), "%s%d/%sxv%d.dat", node_dir.c_str(), inode, z.c_str(), inode);
-        float* xv = nullptr;
-        int64_t np = 0;
-        float hdr[11];
-        rc = asm_read_xv(fname, &xv, &np, hdr);
-        if (rc) {
-            std::fprintf(stderr, "Unable to read xv file: %s\n", fname);
-            asm_destroy(ctx);
-            return 1;
-        }
         float shift[3] = {0, 0, 0};
         {
             char sf[256];
@@ -175,10 +175,32 @@ int main(int argc, char** argv)
                 }
                 std::fclose(fp);
                 std::printf("shift %g %g %g\n", shift[0], shift[1], shift[2]);
-                for (int64_t i = 0; i < np; ++i)
-                    for (int a = 0; a < 3; ++a) xv[6 * i + a] -= shift[a];
             } else {
                 std::printf("No shift file\n");
+            }
+        }
+        float* xv = nullptr;
+        int64_t np = 0;
+        if (read_xv_file) {
+            char fname[512];
+            std::snprintf(fname, sizeof(fname), "%s%d/%sxv%d.dat", node_dir.c_str(), inode, z.c_str(), inode);
+            float hdr[11];
+            rc = asm_read_xv(fname, &xv, &np, hdr);
+            if (rc) {
+                std::fprintf(stderr, "Unable to read xv file: %s\n", fname);
+                asm_destroy(ctx);
+                return 1;
+            }
+            for (int64_t i = 0; i < np; ++i)
+                for (int a = 0; a < 3; ++a) xv[6 * i + a] -= shift[a];
+        } else {
+            float hdr[16];
+            rc = asm_read_zip(node_dir.c_str(), z.c_str(), inode, nc_node_dim, mesh_scale, shift, &xv, &np, hdr);
+            if (rc) {
+                std::fprintf(stderr, "Unable to read zip files %s%d/%szip*_%d.dat (%d)\n", node_dir.c_str(), inode,
+                             z.c_str(), inode, rc);
+                asm_destroy(ctx);
+                return 1;
             }
         }
         std::printf("np_local %lld\n", static_cast<long long>(np));

@@ -198,6 +198,15 @@ double asm_halo_delta_vir(double z, double omega_m);
 /* cubep3m xv reader (file_reader2.cpp:24-86): header = int32 np + 11 float32,
  * then np x {x,y,z,vx,vy,vz}.  *xv is malloc'ed; free with asm_free. */
 int asm_read_xv(const char* path, float** xv, int64_t* np, float header[11]);
+/* cubep3m zip reader (file_reader2.cpp:90-272): the four files
+ * <filebase><inode>/<redshift>zip{0,1,2,3}_<inode>.dat of one node
+ * (per-particle cell offsets in 1/256ths, int16 velocities, per-cell counts
+ * with 255 escaping to int32) decoded into np x {x,y,z,vx,vy,vz}:
+ * x = mesh_scale ((b + 0.5) / 256 + i - 1) - shift, v = s / v_r2i, in FP32.
+ * The node cube is mesh_scale * nc_node_dim.  header: the 16 words of zip0's
+ * header (np_local, a, t, tau, ..., v_r2i at word 12).  *xv is malloc'ed. */
+int asm_read_zip(const char* filebase, const char* redshift, int inode, int nc_node_dim, int mesh_scale,
+                 const float shift[3], float** xv, int64_t* np, float header[16]);
 int asm_write_xv(const char* path, const float* xv, int64_t np, const float header[11]);
 void asm_free(void* p);
 /* The four files of write_mesh_separate (coarse_mesh.cpp:173-201) from a mesh

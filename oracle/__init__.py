@@ -314,3 +314,34 @@ def ref_tool(name: str) -> str:
     _ref/reformat2), or '' when the reference tree was absent at build time."""
     p = os.path.join(HERE, "_ref", name)
     return p if os.path.exists(p) else ""
+
+
+# ---------------------------------------------------------- zip particles ---
+
+def read_zip(filebase: str, redshift: str, inode: int, nc_node_dim: int, mesh_scale: int, shift):
+    """cubep3m zip reader (file_reader2.cpp:90-272) restated with numpy:
+    (xv (np, 6) float32, np_local).  Counts per coarse cell from zip2 (i
+    fastest), 255 escaping to the next int32 of zip3; the particles of a cell
+    are consecutive in zip0 (3 offset bytes) and zip1 (3 int16); x = ms *
+    ((b + 0.5) / 256 + i - 1) - shift, v = s / v_r2i, each step in float32."""
+    path = lambda z: f"{filebase}{inode}/{redshift}zip{z}_{inode}.dat"
+    z0 = np.fromfile(path(0), np.uint8)
+    z1 = np.fromfile(path(1), np.uint8)
+    z2 = np.fromfile(path(2), np.uint8)
+    z3 = np.fromfile(path(3), np.uint8)
+    n = int(z0[:4].view(np.int32)[0])
+    v_r2i = z0[:64].view(np.float32)[12]
+    nd = nc_node_dim
+    cnt = z2[:nd ** 3].astype(np.int64)
+    esc = np.nonzero(cnt == 255)[0]
+    cnt[esc] = z3[:4 * len(esc)].view(np.int32)
+    assert cnt.sum() == n and len(z0) == 64 + 3 * n and len(z1) == 64 + 6 * n
+    cell = np.repeat(np.arange(nd ** 3), cnt)
+    ijk = np.stack([cell % nd + 1, (cell // nd) % nd + 1, cell // (nd * nd) + 1], 1).astype(np.float32)
+    b = z0[64:].reshape(n, 3).astype(np.float32)
+    s = z1[64:].view(np.int16).reshape(n, 3).astype(np.float32)
+    f32 = np.float32
+    frac = (b + f32(0.5)) / f32(256.0)
+    x = f32(mesh_scale) * ((frac + ijk) - f32(1.0)) - np.asarray(shift, np.float32)
+    v = s / v_r2i
+    return np.concatenate([x, v], 1).astype(np.float32), n

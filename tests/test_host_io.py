@@ -203,3 +203,97 @@ def test_reformat_missing_node_file_is_an_error(tmp_path, monkeypatch):
         A.reformat(2, "out", "zc", 1)
     with pytest.raises(A.AsmoothError):
         A.reformat2(2, "out", "z", ".")
+
+
+# ------------------------------------------------------ cubep3m zip files ---
+
+def _write_zip(d, z, inode, nd, counts, b, s, v_r2i):
+    """The four zip files of one node from per-cell counts (i fastest), offset
+    bytes (n, 3) and int16 velocities (n, 3), in cubep3m's layout."""
+    n = int(counts.sum())
+    h0 = np.zeros(16, np.float32)
+    h0[12] = v_r2i
+    h0 = h0.view(np.int32)
+    h0[0] = n
+    os.makedirs(d / f"node{inode}", exist_ok=True)
+    base = d / f"node{inode}" / f"{z}zip"
+    with open(f"{base}0_{inode}.dat", "wb") as f:
+        h0.tofile(f)
+        np.asarray(b, np.uint8).tofile(f)
+    with open(f"{base}1_{inode}.dat", "wb") as f:
+        np.arange(16, dtype=np.float32).tofile(f)
+        np.asarray(s, np.int16).tofile(f)
+    c8 = np.minimum(counts, 255).astype(np.uint8)
+    c8.tofile(f"{base}2_{inode}.dat")
+    counts[counts >= 255].astype(np.int32).tofile(f"{base}3_{inode}.dat")
+
+
+def _zip_case(seed, nd, big=()):
+    rng = np.random.default_rng(seed)
+    counts = rng.poisson(3.0, nd ** 3).astype(np.int64)
+    for c, k in big:
+        counts[c] = k
+    n = int(counts.sum())
+    b = rng.integers(0, 256, (n, 3))
+    s = rng.integers(-32768, 32768, (n, 3))
+    return counts, b, s
+
+
+import os  # noqa: E402
+
+
+def test_zip_oracle_closed_forms(tmp_path):
+    # one particle per cell at offset byte 255 and one at 0: positions are the
+    # cell's corners plus (255.5/256) and (0.5/256) of a cell, exactly
+    nd, ms = 3, 4
+    counts = np.full(nd ** 3, 2, np.int64)
+    b = np.tile(np.array([[255, 255, 255], [0, 0, 0]]), (nd ** 3, 1))
+    s = np.tile(np.array([[1024, -2048, 0], [-1, 1, 32767]]), (nd ** 3, 1))
+    _write_zip(tmp_path, "0.000", 0, nd, counts.copy(), b, s, 512.0)
+    xv, n = O.read_zip(str(tmp_path / "node"), "0.000", 0, nd, ms, (0.0, 0.0, 0.0))
+    assert n == 2 * nd ** 3
+    cell = np.repeat(np.arange(nd ** 3), 2)
+    corner = ms * np.stack([cell % nd, (cell // nd) % nd, cell // nd ** 2], 1)
+    off = np.where((np.arange(n) % 2 == 0)[:, None], 255.5 / 256 * ms, 0.5 / 256 * ms)
+    assert np.array_equal(xv[:, :3], (corner + off).astype(np.float32))
+    assert np.array_equal(xv[0, 3:], np.float32([2.0, -4.0, 0.0]))
+    assert np.array_equal(xv[1, 3:], np.float32([-1 / 512, 1 / 512, 32767 / 512]))
+
+
+@pytest.mark.parametrize("seed,nd,ms,big", [(0, 4, 4, ()), (1, 5, 2, ((3, 255), (7, 300), (60, 1000))),
+                                            (2, 3, 8, ((0, 254), (26, 256)))])
+def test_zip_reader_matches_oracle(tmp_path, seed, nd, ms, big):
+    counts, b, s = _zip_case(seed, nd, big)
+    _write_zip(tmp_path, "7.221", 0, nd, counts.copy(), b, s, 37.5)
+    shift = (0.25, -1.5, 3.0)
+    xv, hdr = A.read_zip(str(tmp_path / "node"), "7.221", 0, nd, ms, shift)
+    ref, n = O.read_zip(str(tmp_path / "node"), "7.221", 0, nd, ms, shift)
+    assert len(xv) == n == counts.sum()
+    assert int(hdr[:1].view(np.int32)[0]) == n and hdr[12] == np.float32(37.5)
+    assert np.array_equal(xv.view(np.int32), ref.view(np.int32))
+    # every particle lies in its coarse cell
+    cell = np.repeat(np.arange(nd ** 3), counts)
+    ijk = np.stack([cell % nd, (cell // nd) % nd, cell // nd ** 2], 1)
+    u = xv[:, :3] + np.float32(shift)
+    assert np.all((u > ms * ijk) & (u < ms * (ijk + 1)))
+
+
+def test_zip_reader_rejects_inconsistent_files(tmp_path):
+    nd = 3
+    counts, b, s = _zip_case(4, nd)
+    _write_zip(tmp_path, "1.0", 0, nd, counts.copy(), b, s, 1.0)
+    A.read_zip(str(tmp_path / "node"), "1.0", 0, nd)
+    p0 = tmp_path / "node0" / "1.0zip0_0.dat"
+    good = p0.read_bytes()
+    p0.write_bytes(good + b"\0")            # zip0 must end after the last particle
+    with pytest.raises(A.AsmoothError):
+        A.read_zip(str(tmp_path / "node"), "1.0", 0, nd)
+    p0.write_bytes(good[:-3])               # a particle short
+    with pytest.raises(A.AsmoothError):
+        A.read_zip(str(tmp_path / "node"), "1.0", 0, nd)
+    p0.write_bytes(good)
+    with pytest.raises(A.AsmoothError):     # wrong node dimension: counts run out
+        A.read_zip(str(tmp_path / "node"), "1.0", 0, nd + 1)
+    (tmp_path / "node0" / "1.0zip3_0.dat").unlink()
+    with pytest.raises(A.AsmoothError):     # zip3 must exist, as in the reference
+        A.read_zip(str(tmp_path / "node"), "1.0", 0, nd)
