@@ -55,11 +55,12 @@ enum TimerId {
     T_KNN_BOUND,
     T_KNN_COLLECT,
     T_KNN_FIXUP,
+    T_STITCH,
     T_COUNT
 };
 const char* kTimerNames[T_COUNT] = {"sort",      "tree",      "knn",       "dens",        "deposit_lists",
                                     "deposit_tiles", "mesh_reduce", "normalize", "mark_halo", "seed",
-                                    "knn_bound", "knn_collect", "knn_fixup"};
+                                    "knn_bound", "knn_collect", "knn_fixup", "stitch"};
 
 struct MeshDev {
     DevBuf<float> mesh;
@@ -817,6 +818,7 @@ int asm_stitch_mesh(asm_context* ctx, int nc, int nn, const int coord[3], int fi
     auto it = ctx->meshes.find(nc);
     if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
     HIPC(hipSetDevice(ctx->device));
+    StageTimer t(ctx, T_STITCH);
     HIPC(launch_stitch(it->second.mesh.p, nc, nn, coord, field, dst, ctx->stream));
     return ASM_OK;
 }

@@ -791,36 +791,38 @@ __global__ void normalize_kernel(float* __restrict__ mesh, int64_t nmesh, float 
 // at cell offset (ox, oy, oz), x fastest, as the stitching tools lay it out
 // (tools/reformat.cpp:81-107).  Fields: 0 n, 1 halo, 2 c, 3 vel (3
 // components interleaved, as in the node mesh), 4 ntot = h + n and 5 the
-// momentum vel * (n / 8) (tools/reformat2.cpp:118-126).  One thread per
-// output float, so consecutive threads write consecutive words of a row.
-__global__ void stitch_kernel(const float* __restrict__ mesh, int nc, int64_t G, int ox, int oy, int oz, int field,
-                              float* __restrict__ dst)
+// momentum vel * (n / 8) (tools/reformat2.cpp:118-126).  A row of nc cells
+// is contiguous in both arrays, so a workgroup copies whole rows: the row's
+// offsets are computed once and consecutive threads move consecutive words.
+template <int kField>
+__global__ __launch_bounds__(256) void stitch_kernel(const float* __restrict__ mesh, int nc, int64_t G, int ox,
+                                                     int oy, int oz, float* __restrict__ dst)
 {
-    const int comps = (field == 3 || field == 5) ? 3 : 1;
+    constexpr int kComps = (kField == 3 || kField == 5) ? 3 : 1;
     const int64_t nmesh = static_cast<int64_t>(nc) * nc * nc;
-    const int64_t total = nmesh * comps;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
-        const int64_t cell = t / comps;
-        const int64_t a = t - cell * comps;
-        const int64_t lx = cell % nc;
-        const int64_t ly = (cell / nc) % nc;
-        const int64_t lz = cell / (static_cast<int64_t>(nc) * nc);
-        const int64_t g = ((oz + lz) * G + (oy + ly)) * G + (ox + lx);
-        float v;
-        if (field == 0)
-            v = mesh[cell];
-        else if (field == 1)
-            v = mesh[nmesh + cell];
-        else if (field == 2)
-            v = mesh[2 * nmesh + cell];
-        else if (field == 3)
-            v = mesh[3 * nmesh + t];
-        else if (field == 4)
-            v = mesh[nmesh + cell] + mesh[cell];
-        else
-            v = mesh[3 * nmesh + t] * (mesh[cell] / 8.0f);
-        dst[g * comps + a] = v;
+    const int64_t rows = static_cast<int64_t>(nc) * nc;
+    const int rowlen = nc * kComps;
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int64_t ly = r % nc;
+        const int64_t lz = r / nc;
+        const int64_t c0 = r * nc;   // node cell (0, ly, lz)
+        float* const out = dst + (((oz + lz) * G + (oy + ly)) * G + ox) * kComps;
+        for (int e = threadIdx.x; e < rowlen; e += blockDim.x) {
+            float v;
+            if (kField == 0)
+                v = mesh[c0 + e];
+            else if (kField == 1)
+                v = mesh[nmesh + c0 + e];
+            else if (kField == 2)
+                v = mesh[2 * nmesh + c0 + e];
+            else if (kField == 3)
+                v = mesh[3 * nmesh + 3 * c0 + e];
+            else if (kField == 4)
+                v = mesh[nmesh + c0 + e] + mesh[c0 + e];
+            else
+                v = mesh[3 * nmesh + 3 * c0 + e] * (mesh[c0 + e / 3] / 8.0f);
+            out[e] = v;
+        }
     }
 }
 
@@ -912,11 +914,18 @@ hipError_t launch_normalize(float* mesh, int nc, float vol, hipStream_t s)
 
 hipError_t launch_stitch(const float* mesh, int nc, int nn, const int coord[3], int field, float* dst, hipStream_t s)
 {
-    const int64_t total = static_cast<int64_t>(nc) * nc * nc * ((field == 3 || field == 5) ? 3 : 1);
-    const int64_t want = (total + 255) / 256;
-    const unsigned blocks = static_cast<unsigned>(want < 65536 ? want : 65536);
-    stitch_kernel<<<blocks, 256, 0, s>>>(mesh, nc, static_cast<int64_t>(nc) * nn, coord[0] * nc, coord[1] * nc,
-                                         coord[2] * nc, field, dst);
+    const int64_t rows = static_cast<int64_t>(nc) * nc;
+    const unsigned blocks = static_cast<unsigned>(rows < (1 << 20) ? rows : (1 << 20));
+    const int64_t G = static_cast<int64_t>(nc) * nn;
+    const int ox = coord[0] * nc, oy = coord[1] * nc, oz = coord[2] * nc;
+    switch (field) {
+    case 0: stitch_kernel<0><<<blocks, 256, 0, s>>>(mesh, nc, G, ox, oy, oz, dst); break;
+    case 1: stitch_kernel<1><<<blocks, 256, 0, s>>>(mesh, nc, G, ox, oy, oz, dst); break;
+    case 2: stitch_kernel<2><<<blocks, 256, 0, s>>>(mesh, nc, G, ox, oy, oz, dst); break;
+    case 3: stitch_kernel<3><<<blocks, 256, 0, s>>>(mesh, nc, G, ox, oy, oz, dst); break;
+    case 4: stitch_kernel<4><<<blocks, 256, 0, s>>>(mesh, nc, G, ox, oy, oz, dst); break;
+    default: stitch_kernel<5><<<blocks, 256, 0, s>>>(mesh, nc, G, ox, oy, oz, dst); break;
+    }
     return hipGetLastError();
 }
 

@@ -247,84 +247,73 @@ int asm_read_zip(const char* filebase, const char* redshift, int inode, int nc_n
                  const float shift[3], float** xv, int64_t* np, float header[16])
 {
     if (!filebase || !redshift || !xv || !np || nc_node_dim <= 0 || mesh_scale <= 0) return ASM_ERR_ARG;
-    FILE* fp[4] = {nullptr, nullptr, nullptr, nullptr};
-    auto close_all = [&]() {
-        for (FILE*& f : fp)
-            if (f) {
-                std::fclose(f);
-                f = nullptr;
-            }
-    };
+    // The four files are read whole, then decoded from memory.
+    std::vector<unsigned char> zf[4];
     for (int z = 0; z < 4; ++z) {
         std::vector<char> name(std::strlen(filebase) + std::strlen(redshift) + 64);
         std::snprintf(name.data(), name.size(), "%s%d/%szip%d_%d.dat", filebase, inode, redshift, z, inode);
-        fp[z] = std::fopen(name.data(), "rb");
-        if (!fp[z]) {
-            close_all();
-            return ASM_ERR_IO;
+        FILE* fp = std::fopen(name.data(), "rb");
+        if (!fp) return ASM_ERR_IO;
+        bool ok = std::fseek(fp, 0, SEEK_END) == 0;
+        const long size = ok ? std::ftell(fp) : -1;
+        ok = ok && size >= 0 && std::fseek(fp, 0, SEEK_SET) == 0;
+        if (ok) {
+            zf[z].resize(static_cast<size_t>(size));
+            ok = std::fread(zf[z].data(), 1, zf[z].size(), fp) == zf[z].size();
         }
+        std::fclose(fp);
+        if (!ok) return ASM_ERR_IO;
     }
+    if (zf[0].size() < 64 || zf[1].size() < 64) return ASM_ERR_IO;
     int32_t h0[16];
-    float h1[16];
-    if (std::fread(h0, sizeof(int32_t), 16, fp[0]) != 16 || std::fread(h1, sizeof(float), 16, fp[1]) != 16 ||
-        h0[0] < 0) {
-        close_all();
-        return ASM_ERR_IO;
-    }
+    std::memcpy(h0, zf[0].data(), sizeof(h0));
     const int64_t n = h0[0];
+    // zip0 and zip1 end exactly after the last particle (:240-248)
+    if (n < 0 || zf[0].size() != 64 + 3 * static_cast<size_t>(n) || zf[1].size() != 64 + 6 * static_cast<size_t>(n))
+        return ASM_ERR_IO;
     float v_r2i;
     std::memcpy(&v_r2i, &h0[12], sizeof(float));
     const float sh[3] = {shift ? shift[0] : 0.0f, shift ? shift[1] : 0.0f, shift ? shift[2] : 0.0f};
+    const int64_t nd = nc_node_dim;
+    if (static_cast<int64_t>(zf[2].size()) < nd * nd * nd) return ASM_ERR_IO;
     float* out = static_cast<float*>(std::malloc(sizeof(float) * 6 * static_cast<size_t>(n > 0 ? n : 1)));
-    if (!out) {
-        close_all();
-        return ASM_ERR_CAPACITY;
-    }
+    if (!out) return ASM_ERR_CAPACITY;
+    const unsigned char* b = zf[0].data() + 64;
+    const unsigned char* vs = zf[1].data() + 64;
+    const unsigned char* c8 = zf[2].data();
+    size_t esc = 0;   // next int32 of zip3
     const float ms = static_cast<float>(mesh_scale);
     int64_t k_out = 0;
     int rc = ASM_OK;
-    const int nd = nc_node_dim;
-    for (int k = 1; k <= nd && rc == ASM_OK; ++k)
-        for (int j = 1; j <= nd && rc == ASM_OK; ++j)
-            for (int i = 1; i <= nd; ++i) {
-                unsigned char c8;
-                if (std::fread(&c8, 1, 1, fp[2]) != 1) {
-                    rc = ASM_ERR_IO;
-                    break;
-                }
-                int32_t cnt = c8;
-                if (cnt == 255 && std::fread(&cnt, sizeof(int32_t), 1, fp[3]) != 1) {
-                    rc = ASM_ERR_IO;
-                    break;
+    for (int64_t k = 1; k <= nd && rc == ASM_OK; ++k)
+        for (int64_t j = 1; j <= nd && rc == ASM_OK; ++j)
+            for (int64_t i = 1; i <= nd; ++i) {
+                int32_t cnt = *c8++;
+                if (cnt == 255) {
+                    if (zf[3].size() < 4 * (esc + 1)) {
+                        rc = ASM_ERR_IO;
+                        break;
+                    }
+                    std::memcpy(&cnt, zf[3].data() + 4 * esc, sizeof(int32_t));
+                    ++esc;
                 }
                 if (cnt < 0 || k_out + cnt > n) {
                     rc = ASM_ERR_IO;
                     break;
                 }
                 const float cell[3] = {static_cast<float>(i), static_cast<float>(j), static_cast<float>(k)};
-                for (int32_t l = 0; l < cnt; ++l) {
-                    unsigned char b[3];
+                for (int32_t l = 0; l < cnt; ++l, ++k_out) {
                     int16_t v[3];
-                    if (std::fread(b, 1, 3, fp[0]) != 3 || std::fread(v, sizeof(int16_t), 3, fp[1]) != 3) {
-                        rc = ASM_ERR_IO;
-                        break;
-                    }
+                    std::memcpy(v, vs + 6 * k_out, sizeof(v));
                     float* o = out + 6 * k_out;
                     for (int a = 0; a < 3; ++a) {
-                        const float frac = (static_cast<float>(b[a]) + 0.5f) / 256.0f;
+                        const float frac = (static_cast<float>(b[3 * k_out + a]) + 0.5f) / 256.0f;
                         o[a] = ms * ((frac + cell[a]) - 1.0f) - sh[a];
                         o[3 + a] = static_cast<float>(v[a]) / v_r2i;
                     }
-                    ++k_out;
                 }
-                if (rc != ASM_OK) break;
             }
-    if (rc == ASM_OK) {
-        unsigned char extra;
-        if (k_out != n || std::fread(&extra, 1, 1, fp[0]) != 0 || std::fread(&extra, 1, 1, fp[1]) != 0)
-            rc = ASM_ERR_IO;
-    }
-    close_all();
+    if (rc == ASM_OK && k_out != n) rc = ASM_ERR_IO;
     if (rc != ASM_OK) {
         std::free(out);
         return rc;
