@@ -48,6 +48,7 @@ OPT_TILE = 4
 OPT_DETERMINISTIC = 5
 OPT_KNN_METHOD = 6
 OPT_DEPOSIT_ACCUM = 7
+OPT_SEED_WALK = 8
 # fields of asm_stitch_mesh (FIELD_NTOT and FIELD_MOMENTUM as tools/reformat2.cpp derives them)
 FIELD_N, FIELD_HALO, FIELD_C, FIELD_VEL, FIELD_NTOT, FIELD_MOMENTUM = range(6)
 FIELD_COMPONENTS = (1, 1, 1, 3, 1, 3)

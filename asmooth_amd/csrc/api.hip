@@ -112,6 +112,7 @@ struct asm_context {
     // learned seed radius
     bool seed_on = false;
     float seed_margin = 0.0f;
+    int seed_walk = 0;   // ASM_OPT_SEED_WALK
     DevBuf<float> seed_params, seed_grid, seed_rcell, tau_init;
     DevBuf<uint32_t> seed_redo;
     DevBuf<unsigned long long> seed_fb;
@@ -317,6 +318,7 @@ int asm_set_option(asm_context* ctx, int option, int value)
         ctx->have_search = false;
         return ASM_OK;
     case ASM_OPT_DETERMINISTIC: ctx->deterministic = value ? 1 : 0; return ASM_OK;
+    case ASM_OPT_SEED_WALK: ctx->seed_walk = value ? 1 : 0; return ASM_OK;
     case ASM_OPT_KNN_METHOD:
         if (value != kKnnBoundCollect && value != kKnnInsertion)
             return fail(ctx, ASM_ERR_ARG, "kNN method %d is not 0 or 1", value);
@@ -456,6 +458,7 @@ int asm_neighbor_search(asm_context* ctx, int k)
         kscr.tau_init = ctx->tau_init.p;
         kscr.seed_redo = ctx->seed_redo.p;
         kscr.seed_fallbacks = ctx->seed_fb.p;
+        kscr.seed_walk = ctx->seed_walk;
         ctx->seed_used = true;
     }
     int32_t* nbr = nullptr;

@@ -80,6 +80,9 @@ struct KnnScratch {
     // by the bound walk to skip groups of an opened leaf; null when the leaf
     // size is not a multiple of kGroup.
     const Box* groups;
+    // With tau_init: 1 runs walk 1 as a range walk within the learned bounds
+    // (knn_seed_walk_kernel), 0 as the bounded walk 1.
+    int seed_walk;
 };
 // Optional marks around the kernels of the bound/collect method (stage 0:
 // bound walk, 1: collect walk, 2: insertion kernel for flagged wavefronts).

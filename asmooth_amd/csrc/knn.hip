@@ -818,6 +818,165 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
     }
 }
 
+// Walk 1 with a learned bound, as a range walk.  Each lane's learned bound
+// tau_init is an upper estimate of its 33rd squared distance; the walk
+// evaluates only the (lane, particle) pairs of leaves whose box distance is
+// within the lane's current bound, packed onto the wavefront as in the
+// collect walk (lane l tests particle l % P of the leaf for the
+// participating lanes of rank l / P, l / P + G, ...).  Accepted squared
+// distances go to the owner's list in LDS; a drain inserts every lane's list
+// into its 33 registers (med3_insert, as walk 1) and lowers the lane's bound
+// to t[32].  A drain runs before any leaf that could overflow a list and at
+// the end, so a list never holds more than kSeedSlots entries.  Every
+// particle with d2 below the final t[32] lies in a leaf within every bound the
+// lane had, so it was inserted: when t[32] < tau_init the 33 smallest are
+// exact; otherwise the wavefront is flagged in seed_redo for walk 1 without a
+// bound, as with the bounded walk 1.  Leaves outside the seed range of the
+// collect walk are recorded for it, as walk 1 records them.
+template <int P>
+struct SeedLds {
+    static constexpr int kSlots = P + 16;
+    float list[kWave][kSlots + 1];   // row stride kSlots + 1 keeps the rows in distinct banks
+    uint32_t cnt[kWave];
+    float4 qtab[kWave];              // participating lanes of a leaf: (x, y, z, bound) ...
+    uint32_t otab[kWave];            // ... and their lane ids
+};
+
+template <int K, int P>
+__global__ __launch_bounds__(kCollectThreads) void knn_seed_walk_kernel(
+    const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
+    int leaf_size, int init_leaves, const float* __restrict__ tau_init, float* __restrict__ tau_out,
+    uint32_t* __restrict__ wleaf, uint32_t* __restrict__ wleaf_n, uint32_t* __restrict__ seed_redo,
+    unsigned long long* __restrict__ seed_fallbacks, KnnStats* __restrict__ stats)
+{
+    constexpr int G = kWave / P;
+    constexpr int C = SeedLds<P>::kSlots;
+    __shared__ SeedLds<P> lds[kCollectWaves];
+    const int lane = threadIdx.x % kWave;
+    const int wid = wave_in_block();
+    SeedLds<P>& L = lds[wid];
+    const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kCollectWaves + wid) * kWave;
+    if (wave0 >= n) return;
+    const int64_t wave = wave0 / kWave;
+    const int64_t q = wave0 + lane;
+    const bool valid = q < n;
+    const float4 me = spos[valid ? q : n - 1];
+    const float qx = me.x, qy = me.y, qz = me.z;
+    // A past-the-end lane has bound -1: box distances are >= 0, so it takes
+    // part in nothing.
+    const float t0 = valid ? tau_init[q] : -1.0f;
+    float t[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) t[i] = t0;
+    float bound = t0;
+    L.cnt[lane] = 0u;
+    wave_lds_fence();
+
+    const int j = lane % P;
+    const int grp = lane / P;
+    const int64_t last = (n < wave0 + kWave ? n : wave0 + kWave) - 1;
+    int64_t l0 = wave0 / leaf_size - init_leaves;
+    int64_t l1 = last / leaf_size + init_leaves;
+    if (l0 < 0) l0 = 0;
+    if (l1 > nleaf - 1) l1 = nleaf - 1;
+    uint32_t rec = 0;
+    int n_rec = 0;
+    uint32_t* const my_leaves = wleaf + wave * kLeafCap;
+    unsigned long long n_leaf = 0, n_pairs = 0, n_node = 0;
+
+    // Insert every lane's list into its registers and lower its bound.
+    const auto drain = [&]() {
+        const int mine = static_cast<int>(L.cnt[lane]);
+        int mx = mine;
+        for (int o = kWave / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+        for (int i = 0; i < mx; ++i) med3_insert<K>(t, i < mine ? L.list[lane][i] : INFINITY);
+        bound = t[K - 1];
+        wave_lds_fence();
+        L.cnt[lane] = 0u;
+        wave_lds_fence();
+    };
+
+    int stack = (lane == 0) ? 1 : 0;
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int64_t node = __builtin_amdgcn_readlane(stack, sp);
+        ++n_node;
+        if (node < nleaf_pad) {
+            // Near child (by lane vote) on top, visited first; a child is
+            // pushed only when some lane's bound reaches its box.
+            const Box b0 = nodes[2 * node];
+            const Box b1 = nodes[2 * node + 1];
+            const float d0 = box_d2(b0, qx, qy, qz);
+            const float d1 = box_d2(b1, qx, qy, qz);
+            const bool need0 = wave_any(d0 <= bound);
+            const bool need1 = wave_any(d1 <= bound);
+            const bool near0 = 2 * __popcll(wave_ballot(d0 <= d1)) >= kWave;
+            const int nearc = static_cast<int>(near0 ? 2 * node : 2 * node + 1);
+            const bool need_near = near0 ? need0 : need1;
+            const bool need_far = near0 ? need1 : need0;
+            stack = (lane == sp) ? (need_far ? (nearc ^ 1) : nearc) : stack;
+            stack = (lane == sp + 1) ? nearc : stack;
+            sp += static_cast<int>(need_far) + static_cast<int>(need_near);
+            continue;
+        }
+        const Box b = nodes[node];
+        const int start = b.lo_w;
+        const int count = b.hi_w;
+        if (!wave_any(box_d2(b, qx, qy, qz) <= bound)) continue;
+        // Room for one more entry per particle of the leaf in every list.
+        if (wave_any(static_cast<int>(L.cnt[lane]) > C - count)) drain();
+        const uint64_t mask = wave_ballot(box_d2(b, qx, qy, qz) <= bound);
+        if (mask == 0) continue;
+        const int64_t leaf = node - nleaf_pad;
+        ++n_leaf;
+        if (leaf < l0 || leaf > l1) {
+            if (n_rec < kLeafCap) {
+                rec = (lane == (n_rec & (kWave - 1))) ? static_cast<uint32_t>(leaf) : rec;
+                if ((n_rec & (kWave - 1)) == kWave - 1) my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
+            }
+            ++n_rec;
+        }
+        const float4 c = (j < count) ? spos[start + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+        const int a = __popcll(mask);
+        n_pairs += static_cast<unsigned long long>(a) * static_cast<unsigned long long>(count);
+        if ((mask >> lane) & 1u) {
+            const int k = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+            L.qtab[k] = make_float4(qx, qy, qz, bound);
+            L.otab[k] = static_cast<uint32_t>(lane);
+        }
+        wave_lds_fence();
+        for (int k = grp; k < a; k += G) {
+            const float4 o = L.qtab[k];
+            const float d2 = norm2(o.x, o.y, o.z, c.x, c.y, c.z);
+            if (d2 < o.w) {
+                const uint32_t own = L.otab[k];
+                const uint32_t slot = atomicAdd(&L.cnt[own], 1u);
+                L.list[own][slot] = d2;
+            }
+        }
+        wave_lds_fence();
+    }
+    drain();
+
+    if (n_rec <= kLeafCap && (n_rec & (kWave - 1)) != 0 && lane < (n_rec & (kWave - 1)))
+        my_leaves[(n_rec & ~(kWave - 1)) + lane] = rec;
+    if (lane == 0) wleaf_n[wave] = static_cast<uint32_t>(n_rec);
+    if (valid) tau_out[q] = t[K - 1];
+    const uint64_t fb = wave_ballot(valid && !(t[K - 1] < t0));
+    if (lane == 0) {
+        seed_redo[wave] = fb ? 1u : 0u;
+        if (fb) atomicAdd(&seed_fallbacks[wave & (kStatSlots - 1)], static_cast<unsigned long long>(__popcll(fb)));
+    }
+    if (stats && lane == 0) {
+        KnnStats* const st = stats + (wave & (kStatSlots - 1));
+        atomicAdd(&st->node_tests, n_node);
+        atomicAdd(&st->leaf_scans, n_leaf);
+        atomicAdd(&st->dist_evals, n_pairs);
+    }
+}
+
 // Largest 33rd squared distance over the particles below each node.
 __global__ void kmax_leaves_kernel(const uint64_t* __restrict__ kkey, int64_t n, int leaf_size, int64_t nleaf,
                                    int64_t nleaf_pad, float* __restrict__ kmax)
@@ -944,10 +1103,23 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     // batches) only when the leaf size is a multiple of kGroup.
     const Box* const groups = (leaf_size % kGroup == 0) ? scratch.groups : nullptr;
     const unsigned bound_blocks = static_cast<unsigned>((waves + kBoundWaves - 1) / kBoundWaves);
+    const unsigned col_blocks = static_cast<unsigned>((waves + kCollectWaves - 1) / kCollectWaves);
     mark_begin(0);
-    knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
-        spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n, stats,
-        scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks, groups);
+    if (scratch.tau_init && scratch.seed_walk) {
+#define ASM_SEED_WALK(PAD)                                                                                        \
+    knn_seed_walk_kernel<kSlots, PAD><<<col_blocks, kCollectThreads, 0, s>>>(                                     \
+        spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau_init, scratch.tau, scratch.wleaf,   \
+        scratch.wleaf_n, scratch.seed_redo, scratch.seed_fallbacks, stats)
+        if (leaf_size <= 8) ASM_SEED_WALK(8);
+        else if (leaf_size <= 16) ASM_SEED_WALK(16);
+        else if (leaf_size <= 32) ASM_SEED_WALK(32);
+        else ASM_SEED_WALK(64);
+#undef ASM_SEED_WALK
+    } else {
+        knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
+            spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
+            stats, scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks, groups);
+    }
     ASM_HIP_CHECK(hipGetLastError());
     if (scratch.tau_init) {
         // Wavefronts with a lane whose learned bound was too small: again, unbounded.
@@ -958,7 +1130,6 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     }
     mark_end(0);
     mark_begin(1);
-    const unsigned col_blocks = static_cast<unsigned>((waves + kCollectWaves - 1) / kCollectWaves);
 #define ASM_COLLECT(M, PAD, ALL)                                                                                \
     knn_collect_kernel<kSlots, M, PAD, ALL><<<col_blocks, kCollectThreads, 0, s>>>(                             \
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.wleaf, scratch.wleaf_n, scratch.tau,  \

@@ -70,8 +70,12 @@ enum {
                                       run to run (default 0: FP64 atomic scatter) */
     ASM_OPT_KNN_METHOD = 6,     /* 0: bound walk + collect walk (default); 1: sorted-key
                                       insertion lists for every wavefront.  Same results. */
-    ASM_OPT_DEPOSIT_ACCUM = 7   /* node-mesh cell sums: 0 FP64 (default); 1 FP32 with Kahan
+    ASM_OPT_DEPOSIT_ACCUM = 7,  /* node-mesh cell sums: 0 FP64 (default); 1 FP32 with Kahan
                                       compensation; 2 plain FP32 atomics (precision study) */
+    ASM_OPT_SEED_WALK = 8       /* with a seed model: 0 (default) starts the bounded walk of
+                                      the plain search from the learned bounds; 1 searches within
+                                      them by a range walk (a study: slower on the 256^3
+                                      workload).  Results are the same. */
 };
 
 const char* asm_version(void);

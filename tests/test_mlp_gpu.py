@@ -54,20 +54,24 @@ def _const_model(b3):
     return A.Mlp(0, p)
 
 
+@pytest.mark.parametrize("walk", [1, 0])
 @pytest.mark.parametrize("b3,margin", [(4.0, 0.0), (-4.0, 0.0), (0.55, 0.05), (0.2, 0.3)])
-def test_seeded_search_is_exact(ctx, b3, margin):
+def test_seeded_search_is_exact(ctx, b3, margin, walk):
     # radius s * e^b3 * (1 + margin): far too large, far too small (every
-    # particle searched again), and near the typical h (some searched again)
+    # particle searched again), and near the typical h (some searched again);
+    # walk 1: range walk within the bounds, walk 0: bounded walk
     xv, L, bf = small_set("zeldovich", 20, 9)
     parts, nl = A.build_buffered_set(xv, L, bf)
     plain = gpu_path(ctx, parts, nl, L, bf, [10])
     model = _const_model(b3)
     ctx.set_seed_model(model, margin)
+    ctx.set_option(A.OPT_SEED_WALK, walk)
     try:
         seeded = gpu_path(ctx, parts, nl, L, bf, [10])
         st = ctx.seed_stats()
     finally:
         ctx.set_seed_model(None)
+        ctx.set_option(A.OPT_SEED_WALK, 0)
     assert np.array_equal(seeded["rk"].view(np.int32), plain["rk"].view(np.int32))
     assert np.array_equal(seeded["nbr"], plain["nbr"])
     assert np.array_equal(seeded["dens"], plain["dens"]) or \
@@ -77,6 +81,25 @@ def test_seeded_search_is_exact(ctx, b3, margin):
         assert st["fallback_particles"] == 0
     if b3 < -3:
         assert st["fallback_particles"] == len(parts)
+
+
+@pytest.mark.parametrize("leaf", [8, 16, 20, 64])
+def test_seeded_range_walk_leaf_sizes(ctx, leaf):
+    xv, L, bf = small_set("blobs", 20, 12)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    plain = gpu_path(ctx, parts, nl, L, bf, [10], leaf_size=leaf)
+    ctx.set_seed_model(_const_model(0.55), 0.1)
+    ctx.set_option(A.OPT_SEED_WALK, 1)
+    try:
+        seeded = gpu_path(ctx, parts, nl, L, bf, [10], leaf_size=leaf)
+        st = ctx.seed_stats()
+    finally:
+        ctx.set_seed_model(None)
+        ctx.set_option(A.OPT_SEED_WALK, 0)
+    assert np.array_equal(seeded["rk"].view(np.int32), plain["rk"].view(np.int32))
+    assert np.array_equal(seeded["nbr"], plain["nbr"])
+    assert np.max(np.abs(seeded["dens"] - plain["dens"]) / plain["dens"]) < 1e-6
+    assert 0 < st["fallback_particles"] < len(parts)
 
 
 def test_seed_samples_and_trained_model(ctx):
