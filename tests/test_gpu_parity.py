@@ -87,6 +87,19 @@ def test_tile_size_does_not_change_mesh(ctx, tile):
         _check_mesh(g["meshes"][nc], o["meshes"][nc], nc, vmax)
 
 
+@pytest.mark.parametrize("tile", [1, 2, 8])
+def test_tiny_meshes_match_oracle(ctx, tile):
+    # one, two and three cells per side: every particle's range is clipped
+    xv, L, bf = small_set("blobs", 10, 13)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    o = oracle_path(buf, L, [1, 2, 3])
+    g = gpu_path(ctx, parts, nl, L, bf, [1, 2, 3], tile=tile)
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    for nc in (1, 2, 3):
+        _check_mesh(g["meshes"][nc], o["meshes"][nc], nc, vmax)
+
+
 def test_minimum_size_and_duplicates(ctx):
     # 40 particles, several exactly coincident: d2 = 0 ties resolved by index
     rng = np.random.default_rng(11)
