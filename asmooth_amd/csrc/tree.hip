@@ -58,7 +58,8 @@ __global__ void gather_sorted_kernel(const ParticleRec* __restrict__ p,
 }
 
 __global__ void tree_leaves_kernel(const float4* __restrict__ spos, int64_t n, int leaf_size,
-                                   int64_t nleaf, int64_t nleaf_pad, Box* __restrict__ nodes)
+                                   int64_t nleaf, int64_t nleaf_pad, const Box* __restrict__ groups,
+                                   Box* __restrict__ nodes)
 {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= nleaf_pad) return;
@@ -68,11 +69,22 @@ __global__ void tree_leaves_kernel(const float4* __restrict__ spos, int64_t n, i
         const int64_t end = (start + leaf_size < n) ? start + leaf_size : n;
         float lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY;
         float hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
-        for (int64_t i = start; i < end; ++i) {
-            const float4 q = spos[i];
-            lo0 = fminf(lo0, q.x); hi0 = fmaxf(hi0, q.x);
-            lo1 = fminf(lo1, q.y); hi1 = fmaxf(hi1, q.y);
-            lo2 = fminf(lo2, q.z); hi2 = fmaxf(hi2, q.z);
+        if (groups) {
+            // The leaf is whole groups (leaf_size is a multiple of kGroup):
+            // the union of their boxes, read as consecutive 32-B records.
+            for (int64_t g = start / kGroup; g <= (end - 1) / kGroup; ++g) {
+                const Box q = groups[g];
+                lo0 = fminf(lo0, q.lo[0]); hi0 = fmaxf(hi0, q.hi[0]);
+                lo1 = fminf(lo1, q.lo[1]); hi1 = fmaxf(hi1, q.hi[1]);
+                lo2 = fminf(lo2, q.lo[2]); hi2 = fmaxf(hi2, q.hi[2]);
+            }
+        } else {
+            for (int64_t i = start; i < end; ++i) {
+                const float4 q = spos[i];
+                lo0 = fminf(lo0, q.x); hi0 = fmaxf(hi0, q.x);
+                lo1 = fminf(lo1, q.y); hi1 = fmaxf(hi1, q.y);
+                lo2 = fminf(lo2, q.z); hi2 = fmaxf(hi2, q.z);
+            }
         }
         b.lo[0] = lo0; b.lo[1] = lo1; b.lo[2] = lo2;
         b.hi[0] = hi0; b.hi[1] = hi1; b.hi[2] = hi2;
@@ -162,8 +174,10 @@ hipError_t launch_build_tree(const float4* spos, int64_t n, int leaf_size, int64
         tree_groups_kernel<<<static_cast<unsigned>((m + th - 1) / th), th, 0, s>>>(spos, n, groups);
         ASM_HIP_CHECK(hipGetLastError());
     }
+    // Group boxes (computed above) stand in for the particles of whole-group leaves.
+    const Box* const leaf_groups = (groups && n > 0 && leaf_size % kGroup == 0) ? groups : nullptr;
     tree_leaves_kernel<<<static_cast<unsigned>((nleaf_pad + th - 1) / th), th, 0, s>>>(
-        spos, n, leaf_size, nleaf, nleaf_pad, nodes);
+        spos, n, leaf_size, nleaf, nleaf_pad, leaf_groups, nodes);
     ASM_HIP_CHECK(hipGetLastError());
     for (int64_t first = nleaf_pad / 2; first >= 1; first /= 2) {
         tree_level_kernel<<<static_cast<unsigned>((first + th - 1) / th), th, 0, s>>>(nodes, first, first);
