@@ -121,7 +121,7 @@ struct asm_context {
     DevBuf<KnnStats> kstats;   // kStatSlots copies
     // deposition
     DevBuf<uint32_t> tile_cnt, tile_start, tile_cur, list;
-    DevBuf<uint64_t> scan_total;
+    DevBuf<uint64_t> scan_total, scan_tmp;
     DevBuf<double> red_partial, red_out;
     std::map<int, MeshDev> meshes;
     // halos
@@ -389,6 +389,7 @@ int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode
     HIPC(ctx->keys_tmp.ensure(n));
     HIPC(ctx->vals_tmp.ensure(n));
     HIPC(ctx->hist.ensure(static_cast<size_t>(radix_hist_entries(n))));
+    HIPC(ctx->scan_tmp.ensure(static_cast<size_t>(scan_scratch_entries(radix_hist_entries(n)))));
     HIPC(ctx->spos.ensure(n));
     HIPC(ctx->sattr.ensure(n));
     HIPC(ctx->inv_perm.ensure(n));
@@ -396,8 +397,8 @@ int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode
         StageTimer t(ctx, T_SORT);
         HIPC(launch_morton_keys(ctx->input.p, n, left, right, ctx->keys.p, ctx->vals.p, s));
         bool in_tmp = false;
-        HIPC(launch_radix_sort(ctx->keys.p, ctx->vals.p, ctx->keys_tmp.p, ctx->vals_tmp.p, n, 30, ctx->hist.p, s,
-                               &in_tmp));
+        HIPC(launch_radix_sort(ctx->keys.p, ctx->vals.p, ctx->keys_tmp.p, ctx->vals_tmp.p, n, 30, ctx->hist.p,
+                               ctx->scan_tmp.p, s, &in_tmp));
         ctx->perm = in_tmp ? ctx->vals_tmp.p : ctx->vals.p;
     }
     ctx->nleaf = (n + ctx->leaf_size - 1) / ctx->leaf_size;
@@ -561,13 +562,14 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     HIPC(ctx->tile_start.ensure(ntiles));
     HIPC(ctx->tile_cur.ensure(ntiles));
     HIPC(ctx->scan_total.ensure(1));
+    HIPC(ctx->scan_tmp.ensure(static_cast<size_t>(scan_scratch_entries(ntiles))));
     uint64_t total = 0;
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cnt.p, ntiles, s));
         HIPC(launch_deposit_count(a, ctx->tile_cnt.p, s));
         HIPC(hipMemcpyAsync(ctx->tile_start.p, ctx->tile_cnt.p, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToDevice, s));
-        HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, s));
+        HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, ctx->scan_tmp.p, s));
     }
     if (!ctx->capturing) {
         HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));

@@ -11,10 +11,14 @@ namespace asm_hip {
 // ---- sort.hip --------------------------------------------------------------
 // In-place exclusive scan of m 32-bit counts; *total (if given) receives
 // their sum in 64 bits, so a caller can tell when the 32-bit offsets wrapped.
-hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint64_t* total, hipStream_t s);
+// With scratch (scan_scratch_entries(m) 64-bit words) long arrays are
+// scanned by many workgroups; without it, by one.
+int64_t scan_scratch_entries(int64_t m);
+hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint64_t* total, uint64_t* scratch, hipStream_t s);
 int64_t radix_hist_entries(int64_t n);
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
-                             int64_t n, int bits, uint32_t* hist, hipStream_t s, bool* result_in_tmp);
+                             int64_t n, int bits, uint32_t* hist, uint64_t* scan_scratch, hipStream_t s,
+                             bool* result_in_tmp);
 
 // ---- tree.hip --------------------------------------------------------------
 // The caller's particle record, byte-compatible with the reference's

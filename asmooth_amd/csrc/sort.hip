@@ -227,12 +227,113 @@ __global__ __launch_bounds__(kScanThreads) void scan_exclusive_1wg(uint32_t* __r
     if (t == 0 && total) *total = carry64_s;
 }
 
+// Multi-workgroup form of the same scan for long arrays: chunks of 4096
+// values get their 64-bit sums (one workgroup each), one workgroup scans the
+// sums, and every chunk is then scanned locally from its offset.  The 32-bit
+// results equal the single-workgroup scan's (sums modulo 2^32).
+constexpr int kScanPer = kSortItems;
+constexpr int kScanChunk = kSortThreads * kScanPer;   // 4096
+
+__global__ __launch_bounds__(kSortThreads) void scan_chunk_sums(const uint32_t* __restrict__ a, int64_t m,
+                                                                uint64_t* __restrict__ sums)
+{
+    __shared__ uint64_t ws[kSortThreads / kWave];
+    const int t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk;
+    uint64_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kScanPer; ++i) {
+        const int64_t idx = base + static_cast<int64_t>(i) * kSortThreads + t;
+        if (idx < m) v += a[idx];
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (t % kWave == 0) ws[t / kWave] = v;
+    __syncthreads();
+    if (t == 0) {
+        uint64_t sum = 0;
+        for (int w = 0; w < kSortThreads / kWave; ++w) sum += ws[w];
+        sums[blockIdx.x] = sum;
+    }
+}
+
+// Exclusive scan of the chunk sums in place (one workgroup); grand total.
+__global__ __launch_bounds__(kScanThreads) void scan_chunk_offsets(uint64_t* __restrict__ sums, int64_t nchunks,
+                                                                   uint64_t* __restrict__ total)
+{
+    __shared__ uint64_t wsum[kScanThreads / kWave];
+    __shared__ uint64_t carry_s;
+    const int t = threadIdx.x;
+    const int lane = t % kWave;
+    const int w = t / kWave;
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < nchunks; c0 += kScanThreads) {
+        const int64_t i = c0 + t;
+        const uint64_t x = i < nchunks ? sums[i] : 0ull;
+        uint64_t inc = x;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint64_t y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == kWave - 1) wsum[w] = inc;
+        __syncthreads();
+        uint64_t pre = 0, tot = 0;
+        for (int ww = 0; ww < kScanThreads / kWave; ++ww) {
+            if (ww < w) pre += wsum[ww];
+            tot += wsum[ww];
+        }
+        const uint64_t carry = carry_s;
+        if (i < nchunks) sums[i] = carry + pre + inc - x;
+        __syncthreads();
+        if (t == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (t == 0 && total) *total = carry_s;
+}
+
+// Each chunk scanned from its offset: thread t holds values 16t .. 16t+15.
+__global__ __launch_bounds__(kSortThreads) void scan_chunk_apply(uint32_t* __restrict__ a, int64_t m,
+                                                                 const uint64_t* __restrict__ offs)
+{
+    __shared__ uint32_t wsum[kSortThreads / kWave];
+    const int t = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanChunk + static_cast<int64_t>(t) * kScanPer;
+    uint32_t x[kScanPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanPer; ++i) {
+        x[i] = (base + i < m) ? a[base + i] : 0u;
+        sum += x[i];
+    }
+    uint32_t run = static_cast<uint32_t>(offs[blockIdx.x]) + block_exclusive_scan_256(sum, wsum, nullptr);
+#pragma unroll
+    for (int i = 0; i < kScanPer; ++i) {
+        if (base + i < m) a[base + i] = run;
+        run += x[i];
+    }
+}
+
 }  // namespace
 
-hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint64_t* total, hipStream_t s)
+int64_t scan_scratch_entries(int64_t m)
+{
+    return (m + kScanChunk - 1) / kScanChunk;
+}
+
+hipError_t launch_scan_exclusive(uint32_t* a, int64_t m, uint64_t* total, uint64_t* scratch, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
-    scan_exclusive_1wg<<<1, kScanThreads, 0, s>>>(a, m, total);
+    if (!scratch || m <= 2 * kScanChunk) {
+        scan_exclusive_1wg<<<1, kScanThreads, 0, s>>>(a, m, total);
+        return hipGetLastError();
+    }
+    const int64_t nchunks = scan_scratch_entries(m);
+    if (nchunks > 0x7fffffffLL) return hipErrorInvalidValue;
+    scan_chunk_sums<<<static_cast<unsigned>(nchunks), kSortThreads, 0, s>>>(a, m, scratch);
+    ASM_HIP_CHECK(hipGetLastError());
+    scan_chunk_offsets<<<1, kScanThreads, 0, s>>>(scratch, nchunks, total);
+    ASM_HIP_CHECK(hipGetLastError());
+    scan_chunk_apply<<<static_cast<unsigned>(nchunks), kSortThreads, 0, s>>>(a, m, scratch);
     return hipGetLastError();
 }
 
@@ -243,7 +344,8 @@ int64_t radix_hist_entries(int64_t n)
 }
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
-                             int64_t n, int bits, uint32_t* hist, hipStream_t s, bool* result_in_tmp)
+                             int64_t n, int bits, uint32_t* hist, uint64_t* scan_scratch, hipStream_t s,
+                             bool* result_in_tmp)
 {
     *result_in_tmp = false;
     if (n <= 1) return hipSuccess;
@@ -255,7 +357,7 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
     for (int shift = 0; shift < bits; shift += 4) {
         rs_hist<<<nb, kSortThreads, 0, s>>>(kin, n, shift, hist, nb);
         ASM_HIP_CHECK(hipGetLastError());
-        ASM_HIP_CHECK(launch_scan_exclusive(hist, static_cast<int64_t>(nb) * kRadix, nullptr, s));
+        ASM_HIP_CHECK(launch_scan_exclusive(hist, static_cast<int64_t>(nb) * kRadix, nullptr, scan_scratch, s));
         rs_scatter<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
         ASM_HIP_CHECK(hipGetLastError());
         uint32_t* tk = kin; kin = kout; kout = tk;
