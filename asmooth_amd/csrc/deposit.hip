@@ -526,7 +526,7 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
         const uint32_t e = static_cast<uint32_t>(lane) * nb + b;
         // Entry setup: box of tile cells [b0, b0 + n) per axis and weights.
         int size = 0;
-        uint32_t packed = 0;    // b0x | b0y << 4 | b0z << 8 | nx << 12 | ny << 16 | igm << 20
+        uint32_t packed = 0;    // b0x | b0y << 4 | b0z << 8 | nx << 12 | ny << 16 | igm << 20 | nz << 21
         float dens = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
         if (e < count) {
             const int64_t i = list[begin + e];
@@ -554,20 +554,25 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
             }
             size = n3[0] * n3[1] * n3[2];
             packed = static_cast<uint32_t>(b0[0] | (b0[1] << 4) | (b0[2] << 8) | (n3[0] << 12) | (n3[1] << 16) |
-                                           (igm << 20));
+                                           (igm << 20) | (n3[2] << 21));
         }
-        // Exclusive scan of the box sizes.
-        int incl = size;
+        // A lane takes two cells of one box per sweep: slot s of a box holds
+        // cells s and s + half (half = ceil(size / 2)), so the per-sweep work
+        // (marks, ballot, gathers of the entry's fields) is shared by two
+        // cells, and both cell sets of a sweep stay consecutive in x.
+        const int slots = (size + 1) >> 1;
+        // Exclusive scan of the box slot counts.
+        int incl = slots;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
             const int v = __shfl_up(incl, o);
             if (lane >= o) incl += v;
         }
-        const int pre = incl - size;
+        const int pre = incl - slots;
         const int total = __shfl(incl, kWave - 1);
-        int first = -1;  // entry holding the cell just before the sweep
+        int first = -1;  // entry holding the slot just before the sweep
         for (int base = 0; base < total; base += kWave, ++tag) {
-            if (size > 0 && pre >= base && pre < base + kWave) map[pre - base] = tag;
+            if (slots > 0 && pre >= base && pre < base + kWave) map[pre - base] = tag;
             wave_lds_fence_dep();
             const uint64_t sb = __ballot(map[lane] == tag);   // cells where an entry starts
             const int pos = base + lane;
@@ -583,30 +588,36 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
             const float p_vz = __shfl(vz, p);
             if (pos < total) {
                 const int bx = pk & 15, by = (pk >> 4) & 15, bz = (pk >> 8) & 15;
-                const int nx = (pk >> 12) & 15, ny = (pk >> 16) & 15;
+                const int nx = (pk >> 12) & 15, ny = (pk >> 16) & 15, nz = (pk >> 21) & 15;
                 const bool igm = (pk >> 20) & 1;
-                const int off = pos - p_pre;
                 const int nxy = nx * ny;
-                const int iz = static_cast<int>((static_cast<float>(off) + 0.5f) *
-                                                __builtin_amdgcn_rcpf(static_cast<float>(nxy)));
-                const int rem = off - iz * nxy;
-                const int iy = static_cast<int>((static_cast<float>(rem) + 0.5f) *
-                                                __builtin_amdgcn_rcpf(static_cast<float>(nx)));
-                const int ix = rem - iy * nx;
+                const int box = nxy * nz;
                 const float* const wp = W + p * kWStride;
-                // coarse_mesh.cpp:93: ker3 = (wx * wy) * wz
-                float ker3 = wp[ix] * wp[kTileMax + iy];
-                ker3 = ker3 * wp[2 * kTileMax + iz];
-                const int c = (bz + iz) * kPadZ + (by + iy) * kPadY + bx + ix;
-                if (igm) {
-                    atomicAdd(&acc[0][c], static_cast<Acc>(ker3));
-                    atomicAdd(&acc[2][c], static_cast<Acc>(ker3 * p_dens));
-                    atomicAdd(&acc[3][c], static_cast<Acc>(ker3 * p_vx));
-                    atomicAdd(&acc[4][c], static_cast<Acc>(ker3 * p_vy));
-                    atomicAdd(&acc[5][c], static_cast<Acc>(ker3 * p_vz));
-                } else {
-                    atomicAdd(&acc[1][c], static_cast<Acc>(ker3));
-                }
+                const auto add_cell = [&](int off) {
+                    const int iz = static_cast<int>((static_cast<float>(off) + 0.5f) *
+                                                    __builtin_amdgcn_rcpf(static_cast<float>(nxy)));
+                    const int rem = off - iz * nxy;
+                    const int iy = static_cast<int>((static_cast<float>(rem) + 0.5f) *
+                                                    __builtin_amdgcn_rcpf(static_cast<float>(nx)));
+                    const int ix = rem - iy * nx;
+                    // coarse_mesh.cpp:93: ker3 = (wx * wy) * wz
+                    float ker3 = wp[ix] * wp[kTileMax + iy];
+                    ker3 = ker3 * wp[2 * kTileMax + iz];
+                    const int c = (bz + iz) * kPadZ + (by + iy) * kPadY + bx + ix;
+                    if (igm) {
+                        atomicAdd(&acc[0][c], static_cast<Acc>(ker3));
+                        atomicAdd(&acc[2][c], static_cast<Acc>(ker3 * p_dens));
+                        atomicAdd(&acc[3][c], static_cast<Acc>(ker3 * p_vx));
+                        atomicAdd(&acc[4][c], static_cast<Acc>(ker3 * p_vy));
+                        atomicAdd(&acc[5][c], static_cast<Acc>(ker3 * p_vz));
+                    } else {
+                        atomicAdd(&acc[1][c], static_cast<Acc>(ker3));
+                    }
+                };
+                const int off = pos - p_pre;
+                const int half = (box + 1) >> 1;
+                add_cell(off);
+                if (off + half < box) add_cell(off + half);
             }
         }
         wave_lds_fence_dep();
