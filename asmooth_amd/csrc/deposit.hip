@@ -493,6 +493,7 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
 constexpr int kPackThreads = 512;
 constexpr int kDepWaves = kPackThreads / kWave;
 constexpr int kWStride = 3 * kTileMax + 1;   // odd row stride: entries fall in different banks
+constexpr int kCellsPerLane = 4;             // cells of one box a lane takes per sweep
 
 template <typename Acc>
 __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
         // cells s and s + half (half = ceil(size / 2)), so the per-sweep work
         // (marks, ballot, gathers of the entry's fields) is shared by two
         // cells, and both cell sets of a sweep stay consecutive in x.
-        const int slots = (size + 1) >> 1;
+        const int slots = (size + kCellsPerLane - 1) / kCellsPerLane;
         // Exclusive scan of the box slot counts.
         int incl = slots;
 #pragma unroll
@@ -615,9 +616,11 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
                     }
                 };
                 const int off = pos - p_pre;
-                const int half = (box + 1) >> 1;
+                const int part = (box + kCellsPerLane - 1) / kCellsPerLane;
                 add_cell(off);
-                if (off + half < box) add_cell(off + half);
+#pragma unroll
+                for (int k = 1; k < kCellsPerLane; ++k)
+                    if (off + k * part < box) add_cell(off + k * part);
             }
         }
         wave_lds_fence_dep();
