@@ -477,15 +477,18 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
 // onto lanes.  A wavefront takes 64 list entries at a time (entries `nb`
 // apart, so that they lie far apart in space and their cells rarely
 // coincide): lane l computes entry l's box of tile cells and its per-axis
-// weights (into LDS), an exclusive scan of the box sizes lays the boxes end
-// to end, and the wavefront sweeps that sequence 64 cells at a time.  Lane l
-// takes cell base + l: the entries whose boxes start inside the sweep mark
-// their start in a per-wavefront LDS map (tagged with the sweep, so it never
-// needs clearing), a ballot of the marks gives every lane its entry, and the
-// lane adds the five or one terms of its cell with ds_add_f64.  Lanes of one
-// entry cover consecutive x cells (consecutive LDS words), every lane does
-// useful work whatever the box shapes, and each term is the reference's
-// FP32 product ((wx * wy) * wz, coarse_mesh.cpp:93).
+// weights (into LDS).  A box of `size` cells is cut into kCellsPerLane
+// parts of q = ceil(size / kCellsPerLane) cells, and slot s of the box holds
+// its cells s, s + q, s + 2q, ...  An exclusive scan of the slot counts lays
+// the boxes' slots end to end, and the wavefront sweeps that sequence 64
+// slots at a time.  Lane l takes slot base + l: the entries whose slots
+// start inside the sweep mark their start in a per-wavefront LDS map (tagged
+// with the sweep, so it never needs clearing), a ballot of the marks gives
+// every lane its entry, and the lane adds the five or one terms of each of
+// its cells with ds_add_f64.  The marks, the ballot and the gathers of the
+// entry's fields are shared by the lane's cells; lanes of one entry still
+// cover consecutive cells (consecutive LDS words) in each part, and each
+// term is the reference's FP32 product ((wx * wy) * wz, coarse_mesh.cpp:93).
 //
 // Eight wavefronts share a tile's accumulators: the tile's padded FP64 sums
 // (27 KB) plus 6.5 KB of weights and marks per wavefront let two workgroups,
@@ -557,10 +560,6 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
             packed = static_cast<uint32_t>(b0[0] | (b0[1] << 4) | (b0[2] << 8) | (n3[0] << 12) | (n3[1] << 16) |
                                            (igm << 20) | (n3[2] << 21));
         }
-        // A lane takes two cells of one box per sweep: slot s of a box holds
-        // cells s and s + half (half = ceil(size / 2)), so the per-sweep work
-        // (marks, ballot, gathers of the entry's fields) is shared by two
-        // cells, and both cell sets of a sweep stay consecutive in x.
         const int slots = (size + kCellsPerLane - 1) / kCellsPerLane;
         // Exclusive scan of the box slot counts.
         int incl = slots;
