@@ -1,9 +1,10 @@
-// LSD radix sort of 32-bit keys with 32-bit payloads, and a single-workgroup
-// exclusive scan.  Used to put particles in Morton order (the tree build) and
-// to turn per-tile counts into list offsets (mesh deposition).
+// LSD radix sort of 32-bit keys with 32-bit payloads, and an exclusive scan
+// (one workgroup for short arrays, many for long ones).  Used to put
+// particles in Morton order (the tree build) and to turn per-tile counts into
+// list offsets (mesh deposition).
 //
-// Each pass handles 4 key bits.  A workgroup owns a tile of 4096 keys; thread
-// t owns 16 consecutive keys, so the in-tile order is (thread, item) and the
+// Each pass handles kRadixBits (4) key bits.  A workgroup owns a tile of 4096
+// keys; thread t owns 16 consecutive keys, so the in-tile order is (thread, item) and the
 // per-(digit, thread) counts scanned in digit-major order give every key a
 // stable destination.  Per-tile digit totals are scanned across tiles in
 // digit-major order, which makes each pass stable across the whole array.
@@ -16,7 +17,9 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
-constexpr int kRadix = 16;
+constexpr int kRadixBits = 4;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kDigitThreads = kSortThreads / kRadix;   // threads per digit row in the count scan
 constexpr int kCntStride = kSortThreads + 1;          // padded digit rows
 
 __global__ __launch_bounds__(kSortThreads) void rs_hist(const uint32_t* __restrict__ keys,
@@ -112,10 +115,11 @@ __global__ __launch_bounds__(kSortThreads) void rs_scatter(const uint32_t* __res
         if (j < nvalid) cnt[((k[j] >> shift) & (kRadix - 1)) * kCntStride + t] += 1u;
     __syncthreads();
 
-    // Digit-major exclusive scan of the 16 x 256 counts: thread t scans the
-    // 16 consecutive entries e = 16t .. 16t+15 (digit t/16, threads 16(t%16)+i).
-    const int d0 = t / kRadix;
-    const int t0 = (t % kRadix) * kRadix;
+    // Digit-major exclusive scan of the kRadix x 256 counts: thread t scans
+    // the kRadix consecutive entries e = kRadix t .. kRadix t + kRadix - 1
+    // (digit t / kDigitThreads, threads kRadix (t % kDigitThreads) + i).
+    const int d0 = t / kDigitThreads;
+    const int t0 = (t % kDigitThreads) * kRadix;
     uint32_t s[kRadix];
     uint32_t sum = 0;
 #pragma unroll
@@ -354,7 +358,7 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
     uint32_t* vin = vals;
     uint32_t* kout = keys_tmp;
     uint32_t* vout = vals_tmp;
-    for (int shift = 0; shift < bits; shift += 4) {
+    for (int shift = 0; shift < bits; shift += kRadixBits) {
         rs_hist<<<nb, kSortThreads, 0, s>>>(kin, n, shift, hist, nb);
         ASM_HIP_CHECK(hipGetLastError());
         ASM_HIP_CHECK(launch_scan_exclusive(hist, static_cast<int64_t>(nb) * kRadix, nullptr, scan_scratch, s));
