@@ -239,7 +239,7 @@ def main(argv=None) -> int:
                     help="0: bound walk + collect walk (default); 1: sorted-key insertion lists")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU). "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported. "
               "Per-node shares are independent after one buffer exchange, so N GPUs run N node shares.",
               file=sys.stderr)
         return 2
