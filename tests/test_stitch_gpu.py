@@ -11,8 +11,9 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_device_stitch_matches_oracle_and_reformat2(ctx, tmp_path, monkeypatch):
-    n_side, L, bf, nn, nc = 16, 32.0, 0.1, 2, 6
+@pytest.mark.parametrize("nn,nc", [(2, 6), (3, 5)])
+def test_device_stitch_matches_oracle_and_reformat2(ctx, tmp_path, monkeypatch, nn, nc):
+    n_side, L, bf = 18, 36.0, 0.1
     xv = synth.zeldovich(n=n_side, L=L, rms=1.0, seed=11)
     G = nn * nc
     arrays = {f: A.DeviceArray(G ** 3 * A.FIELD_COMPONENTS[f]) for f in range(6)}
