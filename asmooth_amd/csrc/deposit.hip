@@ -103,48 +103,58 @@ __device__ __forceinline__ PartRange part_range(const DepositArgs& a, int64_t i)
     return r;
 }
 
-// Tiles whose cells a particle's range [lo, hi) touches, per axis, as up to
-// three disjoint tile intervals.  Node mesh: one interval.  Periodic variant:
-// the unwrapped range is cut at multiples of nc, each piece is wrapped into
-// [0, nc) and mapped to tiles, and overlapping intervals are merged so that
-// no tile is listed twice (nc need not be a multiple of the tile side).
+// Tiles whose cells a particle's range [lo, hi) touches, per axis, as one or
+// two disjoint tile intervals.  Node mesh: one interval.  Periodic variant:
+// the unwrapped range (around coord * nc, global_mesh.cpp:43-44, 66-69) is
+// taken modulo nc (:85-87): the whole axis when it spans nc cells or more,
+// else an arc that may wrap past nc - 1 into two pieces, merged into one
+// interval when they meet, so that no tile is listed twice (nc need not be a
+// multiple of the tile side).
 struct AxisTiles {
-    int a[3], b[3];  // inclusive tile intervals
+    int a[2], b[2];  // inclusive tile intervals
     int n;
 };
+
+__device__ __forceinline__ int floor_div(int x, int m)
+{
+    const int q = x / m;
+    return (x % m != 0 && x < 0) ? q - 1 : q;
+}
 
 __device__ __forceinline__ AxisTiles axis_tiles(const DepositArgs& a, int lo, int hi)
 {
     AxisTiles t;
-    t.n = 0;
+    t.n = 1;
     if (!a.periodic) {
         t.a[0] = lo / a.tile;
         t.b[0] = (hi - 1) / a.tile;
-        t.n = 1;
         return t;
     }
-    for (int k = -1; k <= 1; ++k) {
-        const int s0 = max(lo, k * a.nc), s1 = min(hi, (k + 1) * a.nc);
-        if (s0 >= s1) continue;
-        int ta = (s0 - k * a.nc) / a.tile;
-        int tb = (s1 - 1 - k * a.nc) / a.tile;
-        // merge with any interval it overlaps or touches
-        for (int m = 0; m < t.n;) {
-            if (ta <= t.b[m] + 1 && t.a[m] <= tb + 1) {
-                ta = min(ta, t.a[m]);
-                tb = max(tb, t.b[m]);
-                t.a[m] = t.a[t.n - 1];
-                t.b[m] = t.b[t.n - 1];
-                --t.n;
-                m = 0;
-            } else {
-                ++m;
-            }
-        }
-        t.a[t.n] = ta;
-        t.b[t.n] = tb;
-        ++t.n;
+    const int last = a.ntx - 1;
+    if (hi - lo >= a.nc) {
+        t.a[0] = 0;
+        t.b[0] = last;
+        return t;
     }
+    const int l = lo - floor_div(lo, a.nc) * a.nc;   // lo modulo nc, in [0, nc)
+    const int r = l + (hi - lo) - 1;                   // last cell, unwrapped
+    if (r < a.nc) {
+        t.a[0] = l / a.tile;
+        t.b[0] = r / a.tile;
+        return t;
+    }
+    const int b0 = (r - a.nc) / a.tile;   // wrapped piece [0, b0]
+    const int a1 = l / a.tile;            // upper piece [a1, last]
+    if (b0 + 1 >= a1) {
+        t.a[0] = 0;
+        t.b[0] = last;
+        return t;
+    }
+    t.a[0] = 0;
+    t.b[0] = b0;
+    t.a[1] = a1;
+    t.b[1] = last;
+    t.n = 2;
     return t;
 }
 
@@ -420,19 +430,29 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
         const float dens = a.dens[i];
         const int igrp = __float_as_int(at.w);
         const bool igm = (a.sign * igrp <= 0);
-        // Periodic variant: the unwrapped cell range may run past either end of
-        // [0, nc); cell i lands on tile cell i - k*nc - c0 for the shift k that
-        // puts it inside the tile (k in {-1, 0, 1} since h < boxsize).
-        const int kr = a.periodic ? 1 : 0;
-        for (int kz = -kr; kz <= kr; ++kz) {
+        // Periodic variant: the unwrapped cell range lies around coord * nc and
+        // may run past either end of the node's cells; cell i lands on tile
+        // cell i - k*nc - c0 for every shift k that puts it inside the tile
+        // (global_mesh.cpp:85-87 takes i modulo nc).  Per axis those are
+        // k in [kmin, kmax], from [c0 + k nc, c0 + k nc + T) meeting [lo, hi).
+        int kmin[3] = {0, 0, 0}, kmax[3] = {0, 0, 0};
+        if (a.periodic) {
+            const int c0[3] = {cx0, cy0, cz0};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                kmin[d] = floor_div(r.lo[d] - c0[d] - T, a.nc) + 1;
+                kmax[d] = floor_div(r.hi[d] - 1 - c0[d], a.nc);
+            }
+        }
+        for (int kz = kmin[2]; kz <= kmax[2]; ++kz) {
             const int sz = kz * a.nc;
             const int z0 = max(r.lo[2], cz0 + sz), z1 = min(r.hi[2], cz0 + sz + T);
             if (z0 >= z1) continue;
-            for (int ky = -kr; ky <= kr; ++ky) {
+            for (int ky = kmin[1]; ky <= kmax[1]; ++ky) {
                 const int sy = ky * a.nc;
                 const int y0 = max(r.lo[1], cy0 + sy), y1 = min(r.hi[1], cy0 + sy + T);
                 if (y0 >= y1) continue;
-                for (int kx = -kr; kx <= kr; ++kx) {
+                for (int kx = kmin[0]; kx <= kmax[0]; ++kx) {
                     const int sx = kx * a.nc;
                     const int x0 = max(r.lo[0], cx0 + sx), x1 = min(r.hi[0], cx0 + sx + T);
                     if (x0 >= x1) continue;

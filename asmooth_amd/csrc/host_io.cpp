@@ -1,6 +1,7 @@
 // Host-side steps either side of the device path: the buffered particle set
 // of a node, the cubep3m xv reader/writer, the mesh file writer and the
 // stitching of node mesh files into global ones.
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -49,47 +50,58 @@ inline asm_particle make_rec(const float* u, const float* v)
     return p;
 }
 
+// The buffer records of one node in the order its exchange sends them
+// (exchange_buffer.cpp:36-93, exchange_buffer_template.h:38-79): the records
+// of `idx` (the node's own, in file order) outside the central region are
+// taken in reverse order (the copy loop writes them from the end of the
+// allocation down, :36-47), then cycle-sorted in place by region (:60-93).
+// q receives record indices; region r occupies q[begin[r], begin[r] + nbuf[r]).
+template <typename Region>
+void send_order(const std::vector<int64_t>& idx, Region&& region, std::vector<int64_t>& q, int64_t begin[27],
+                int64_t nbuf[27])
+{
+    q.clear();
+    for (int r = 0; r < 27; ++r) nbuf[r] = 0;
+    for (size_t t = idx.size(); t-- > 0;) {
+        const int r = region(idx[t]);
+        if (r != 13) {
+            q.push_back(idx[t]);
+            nbuf[r]++;
+        }
+    }
+    int64_t next[27];
+    next[0] = begin[0] = 0;
+    for (int r = 1; r < 27; ++r) next[r] = begin[r] = begin[r - 1] + nbuf[r - 1];
+    for (int r = 0; r < 27; ++r) {
+        const int64_t end = begin[r] + nbuf[r];
+        for (int64_t j = next[r]; j < end; ++j) {
+            int g = region(q[j]);
+            while (g != r) {
+                const int64_t k = next[g]++;
+                std::swap(q[k], q[j]);
+                g = region(q[j]);
+            }
+        }
+    }
+}
+
 // Local records followed by their periodic images, in the order the
 // reference's single-rank exchange produces them (exchange_buffer.cpp:13-151,
-// exchange_buffer_template.h:18-122): candidates are taken in reverse file
-// order (the copy loop writes them from the end of the allocation down,
-// :36-47), cycle-sorted in place by region (:60-93), then sent per (source
-// region i, direction j) and shifted by -(dn-1) L (:110-151).  emit(rec,
-// shift) receives each record with the shift to subtract from its position
-// (zero for the local copies).
+// exchange_buffer_template.h:18-122): the records in file order, then, per
+// (source region i, direction j) pair that carries region i (:110-151), the
+// region-i records in send order (send_order), shifted by -(dn-1) L.
+// emit(rec, shift) receives each record with the shift to subtract from its
+// position (zero for the local copies).
 template <typename Emit>
 void periodic_images(const float* recs, int width, int64_t n, float boxsize, float bf, Emit&& emit)
 {
     const float a = bf * boxsize;            // exchange_buffer.cpp:23
     const float b = (1.0f - bf) * boxsize;   // exchange_buffer.cpp:24
     const float zero[3] = {0.0f, 0.0f, 0.0f};
-    std::vector<int64_t> q;
-    int64_t nbuf[27] = {0};
-    for (int64_t i = n - 1; i >= 0; --i) {
-        const int r = region_index(recs + width * i, a, b);
-        if (r != 13) {
-            q.push_back(i);
-            nbuf[r]++;
-        }
-    }
-    int64_t next[27], begin[27], end[27];
-    next[0] = begin[0] = 0;
-    end[0] = nbuf[0];
-    for (int i = 1; i < 27; ++i) {
-        next[i] = next[i - 1] + nbuf[i - 1];
-        begin[i] = next[i];
-        end[i] = next[i] + nbuf[i];
-    }
-    for (int i = 0; i < 27; ++i) {
-        for (int64_t j = next[i]; j < end[i]; ++j) {
-            int r = region_index(recs + width * q[j], a, b);
-            while (r != i) {
-                const int64_t k = next[r]++;
-                std::swap(q[k], q[j]);
-                r = region_index(recs + width * q[j], a, b);
-            }
-        }
-    }
+    std::vector<int64_t> idx(static_cast<size_t>(n)), q;
+    for (int64_t i = 0; i < n; ++i) idx[i] = i;
+    int64_t begin[27], nbuf[27];
+    send_order(idx, [&](int64_t i) { return region_index(recs + width * i, a, b); }, q, begin, nbuf);
     for (int64_t i = 0; i < n; ++i) emit(recs + width * i, zero);
     for (int i = 0; i < 27; ++i) {
         for (int j = 0; j < 27; ++j) {
@@ -169,38 +181,68 @@ int64_t asm_build_node_share(const float* xv, int64_t np, float box_total, int n
     const float L = box_total / static_cast<float>(nn);
     const float a = bf * L;
     const float b = (1.0f - bf) * L;
-    std::vector<asm_particle> local;
-    std::vector<std::vector<asm_particle>> images(27 * 27);
+    // Every particle's node and node-local position (the node files of a
+    // cubep3m run hold node-local positions).
+    std::vector<float> u(3 * static_cast<size_t>(np));
+    std::vector<int32_t> node(static_cast<size_t>(np));
     for (int64_t i = 0; i < np; ++i) {
         const float* p = xv + 6 * i;
         int home[3];
-        float u[3];
         for (int d = 0; d < 3; ++d) {
             int h = static_cast<int>(p[d] / L);
             if (h < 0) h = 0;
             if (h >= nn) h = nn - 1;
             home[d] = h;
-            u[d] = p[d] - static_cast<float>(h) * L;
+            u[3 * i + d] = p[d] - static_cast<float>(h) * L;
         }
-        if (home[0] == coord[0] && home[1] == coord[1] && home[2] == coord[2])
-            local.push_back(make_rec(u, p + 3));
-        const int r = region_index(u, a, b);
-        if (r == 13) continue;
+        node[i] = home[0] + nn * (home[1] + nn * home[2]);
+    }
+    const auto node_at = [nn](int x, int y, int z) {
+        return ((x % nn + nn) % nn) + nn * (((y % nn + nn) % nn) + nn * ((z % nn + nn) % nn));
+    };
+    const int me = node_at(coord[0], coord[1], coord[2]);
+    // The nodes that send to this one: the neighbour in direction -dn for
+    // every direction dn (mpi_interface.cpp:31-32, 54-68), each with its own
+    // records in file order.
+    std::vector<int> sender(27, -1);
+    std::vector<int> slot(static_cast<size_t>(nn) * nn * nn, -1);
+    std::vector<std::vector<int64_t>> members;
+    for (int j = 0; j < 27; ++j) {
+        if (j == 13) continue;
+        const int s = node_at(coord[0] - (j % 3 - 1), coord[1] - ((j / 3) % 3 - 1), coord[2] - (j / 9 - 1));
+        if (slot[s] < 0) {
+            slot[s] = static_cast<int>(members.size());
+            members.emplace_back();
+        }
+        sender[j] = slot[s];
+    }
+    std::vector<asm_particle> all;
+    for (int64_t i = 0; i < np; ++i) {
+        if (node[i] == me) all.push_back(make_rec(&u[3 * i], xv + 6 * i + 3));
+        if (slot[node[i]] >= 0) members[slot[node[i]]].push_back(i);
+    }
+    const int64_t nlocal = static_cast<int64_t>(all.size());
+    std::vector<std::vector<int64_t>> q(members.size());
+    std::vector<std::array<int64_t, 27>> begin(members.size()), nbuf(members.size());
+    const auto region = [&](int64_t i) { return region_index(&u[3 * i], a, b); };
+    for (size_t m = 0; m < members.size(); ++m) send_order(members[m], region, q[m], begin[m].data(), nbuf[m].data());
+    // Receives in the exchange's order (exchange_buffer.cpp:112-151): per
+    // source region i and direction j, the sender's region-i records, shifted
+    // by -(dn-1) L.
+    for (int i = 0; i < 27; ++i) {
         for (int j = 0; j < 27; ++j) {
-            if (!direction_carries(r, j)) continue;
-            const int dn[3] = {j % 3 - 1, (j / 3) % 3 - 1, j / 9 - 1};
-            bool to_me = true;
-            for (int d = 0; d < 3; ++d)
-                if (((home[d] + dn[d]) % nn + nn) % nn != coord[d]) to_me = false;
-            if (!to_me) continue;
-            const float w[3] = {u[0] - static_cast<float>(dn[0]) * L, u[1] - static_cast<float>(dn[1]) * L,
-                                u[2] - static_cast<float>(dn[2]) * L};
-            images[r * 27 + j].push_back(make_rec(w, p + 3));
+            if (!direction_carries(i, j)) continue;
+            const int m = sender[j];
+            const float s[3] = {static_cast<float>(j % 3 - 1) * L, static_cast<float>((j / 3) % 3 - 1) * L,
+                                static_cast<float>(j / 9 - 1) * L};
+            for (int64_t k = begin[m][i]; k < begin[m][i] + nbuf[m][i]; ++k) {
+                const int64_t r = q[m][k];
+                const float w[3] = {u[3 * r] - s[0], u[3 * r + 1] - s[1], u[3 * r + 2] - s[2]};
+                all.push_back(make_rec(w, xv + 6 * r + 3));
+            }
         }
     }
-    std::vector<asm_particle> all = local;
-    for (auto& v : images) all.insert(all.end(), v.begin(), v.end());
-    return keep_inside(all, static_cast<int64_t>(local.size()), L, bf, out, cap, np_local);
+    return keep_inside(all, nlocal, L, bf, out, cap, np_local);
 }
 
 int asm_read_xv(const char* path, float** xv, int64_t* np, float header[11])

@@ -312,7 +312,12 @@ def test_normalisation_matches_oracle_fp32(ctx):
     assert np.array_equal(got.view(np.int32), want.view(np.int32))
 
 
-def test_global_mesh_variant_matches_oracle(ctx):
+@pytest.mark.parametrize("coord", [(0, 0, 0), (1, 0, 2), (1, 1, 1), (2, 2, 2), (0, 3, 1)])
+def test_global_mesh_variant_matches_oracle(ctx, coord):
+    """assign_on_global_mesh at node coordinates 0..3: the cell range lies
+    around coord * nc and wraps modulo nc (global_mesh.cpp:43-44, 66-69,
+    85-87); the oracle is pinned against the reference's own function at every
+    coordinate of a 2^3 decomposition (tests/test_reference_mpi.py)."""
     xv, L, bf = small_set("blobs", 16, 12)
     parts, nl = A.build_buffered_set(xv, L, bf)
     buf = np.concatenate([parts["x"], parts["v"]], axis=1)
@@ -321,10 +326,10 @@ def test_global_mesh_variant_matches_oracle(ctx):
     # nc not a multiple of the tile side, and tiles wider than the mesh
     for nc, tile in ((16, 8), (10, 8), (7, 3), (13, 5), (5, 8)):
         ctx.set_option(A.OPT_TILE, tile)
-        ctx.assign_on_global_mesh(nc, -1, (0, 0, 0))
+        ctx.assign_on_global_mesh(nc, -1, coord)
         gm = ctx.get_mesh(nc)
         om = O.assign_on_mesh(buf[:, :3], buf[:, 3:], o["dens32"], o["rk"], np.zeros(len(buf), np.int32), L, nc,
-                              -1, periodic_coord=(0, 0, 0))
+                              -1, periodic_coord=coord)
         _check_mesh(gm, om, nc, float(np.max(np.abs(buf[:, 3:]))))
         # periodic: every local particle's full weight lands on the mesh
         assert abs(mesh_fields(gm, nc)["n"].sum() - nl) < 1e-3 * nl
