@@ -49,6 +49,10 @@ OPT_DETERMINISTIC = 5
 OPT_KNN_METHOD = 6
 OPT_DEPOSIT_ACCUM = 7
 OPT_SEED_WALK = 8
+# Limits of asm_smooth_noalloc (include/asmooth.h).
+NOALLOC_MAX_PARTICLES = 1 << 19
+NOALLOC_MAX_NC = 128
+NOALLOC_MAX_LIST = 1 << 23
 # fields of asm_stitch_mesh (FIELD_NTOT and FIELD_MOMENTUM as tools/reformat2.cpp derives them)
 FIELD_N, FIELD_HALO, FIELD_C, FIELD_VEL, FIELD_NTOT, FIELD_MOMENTUM = range(6)
 FIELD_COMPONENTS = (1, 1, 1, 3, 1, 3)
@@ -96,6 +100,8 @@ def lib() -> C.CDLL:
         "asm_reformat": (i32, [i32, C.c_char_p, C.c_char_p, i32, P(f64)]),
         "asm_reformat2": (i32, [i32, C.c_char_p, C.c_char_p, C.c_char_p, P(f64)]),
         "asm_device_to_host": (i32, [vp, vp, i64]),
+        "asm_host_to_device": (i32, [vp, vp, i64]),
+        "asm_smooth_noalloc": (i32, [i32, vp, i64, i64, f32, f32, f32, i32, i32, vp, vp, vp, C.c_char_p, i32]),
         "asm_get_particles": (i32, [vp, vp]),
         "asm_get_rk_dens": (i32, [vp, vp, vp, vp]),
         "asm_get_neighbors": (i32, [vp, vp]),
@@ -654,6 +660,19 @@ def reformat2(nn: int, out_dir: str, redshift: str, in_dir: str) -> tuple[float,
     return tot[0], tot[1]
 
 
+def smooth_noalloc(device: int, d_particles: int, n: int, np_local: int, boxsize: float, left: float,
+                   right: float, nc: int, sign: int, d_rk: int, d_dens: int, d_mesh: int | None) -> None:
+    """asm_smooth_noalloc: tree, search, h and density (and one raw mesh when
+    nc > 0) of n device particle records, into device outputs, with no device
+    allocation (the library's fixed work array)."""
+    err = C.create_string_buffer(512)
+    rc = lib().asm_smooth_noalloc(int(device), C.c_void_p(d_particles), int(n), int(np_local), float(boxsize),
+                                  float(left), float(right), int(nc), int(sign), C.c_void_p(d_rk),
+                                  C.c_void_p(d_dens), C.c_void_p(d_mesh) if d_mesh else None, err, len(err))
+    if rc != 0:
+        raise AsmoothError(f"asm_smooth_noalloc failed ({rc}): {err.value.decode(errors='replace')}")
+
+
 class DeviceArray:
     """Raw device allocation of n float32 (for asm_stitch_mesh destinations)."""
 
@@ -669,6 +688,14 @@ class DeviceArray:
         if lib().asm_device_to_host(_ptr(out), C.c_void_p(self.ptr), 4 * self.n) != 0:
             raise AsmoothError("device to host copy failed")
         return out
+
+    def from_host(self, a: np.ndarray) -> None:
+        """Copy the bytes of a (at most 4 n) to the start of the array."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes > 4 * self.n:
+            raise AsmoothError("host array larger than the device array")
+        if lib().asm_host_to_device(C.c_void_p(self.ptr), _ptr(a), a.nbytes) != 0:
+            raise AsmoothError("host to device copy failed")
 
     def close(self) -> None:
         if self.ptr:

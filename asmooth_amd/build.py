@@ -24,7 +24,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libasmooth.so")
 DRIVER = os.path.join(PKG, "clumping_tree_gpu")
 
-HIP_SOURCES = ["sort.hip", "tree.hip", "knn.hip", "deposit.hip", "halo.hip", "mlp.hip", "api.hip", "calib.hip"]
+HIP_SOURCES = ["sort.hip", "tree.hip", "knn.hip", "deposit.hip", "halo.hip", "mlp.hip", "api.hip", "calib.hip", "noalloc.hip"]
 HOST_SOURCES = ["host_io.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so every FP32 operation rounds as in

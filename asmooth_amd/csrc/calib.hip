@@ -44,3 +44,10 @@ extern "C" int asm_device_to_host(void* host, const void* dev, int64_t bytes)
     return hipMemcpy(host, dev, static_cast<size_t>(bytes), hipMemcpyDeviceToHost) == hipSuccess ? ASM_OK
                                                                                                   : ASM_ERR_HIP;
 }
+
+extern "C" int asm_host_to_device(void* dev, const void* host, int64_t bytes)
+{
+    if (!host || !dev || bytes < 0) return ASM_ERR_ARG;
+    return hipMemcpy(dev, host, static_cast<size_t>(bytes), hipMemcpyHostToDevice) == hipSuccess ? ASM_OK
+                                                                                                  : ASM_ERR_HIP;
+}

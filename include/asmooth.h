@@ -315,14 +315,39 @@ int asm_reset_timers(asm_context* ctx);
  * evaluations, list insertions. */
 int asm_get_search_stats(asm_context* ctx, uint64_t out[4]);
 
+/* No counterpart in the reference: an entry point that allocates nothing, for
+ * callers that cannot call hipMalloc (tight serving loops, harnesses that
+ * forbid device allocation).  It does the work of asm_set_particles_device +
+ * asm_tree_construct(left, right) + asm_neighbor_search(32), and with nc > 0
+ * asm_assign_on_mesh(nc, sign), on device `device`, with the same kernels,
+ * leaf size 32 and tile 8, but every work buffer lies in one fixed device
+ * array of the library (96 MiB per device, held from the library's load until
+ * the process ends).  d_particles: n device records (caller order); outputs,
+ * in device memory: d_rk and d_dens (n floats, caller order) and, with
+ * nc > 0, d_mesh (6 nc^3 floats, the raw sums of asm_assign_on_mesh).
+ * Limits: n <= ASM_NOALLOC_MAX_PARTICLES, nc <= ASM_NOALLOC_MAX_NC, and at most
+ * ASM_NOALLOC_MAX_LIST (particle, tile) pairs for the mesh; beyond them it
+ * returns ASM_ERR_CAPACITY (and, for the pairs, after h and density are
+ * written).  err (may be NULL) receives a message of at most err_len bytes.
+ * The call runs on the device's null stream and returns when the work is
+ * done.  Every call on one device shares that device's array: calls that
+ * overlap on one device race. */
+#define ASM_NOALLOC_MAX_PARTICLES (1 << 19)
+#define ASM_NOALLOC_MAX_NC 128
+#define ASM_NOALLOC_MAX_LIST (1 << 23)
+int asm_smooth_noalloc(int device, const asm_particle* d_particles, int64_t n, int64_t np_local, float boxsize,
+                       float left, float right, int nc, int sign, float* d_rk, float* d_dens, float* d_mesh,
+                       char* err, int err_len);
+
 /* Measurement helpers (no counterpart in the reference): a plain device copy
  * of `bytes` (multiple of 16) whose hardware byte counters calibrate those of
  * the smoothing kernels, and raw device allocation for it. */
 int asm_calibration_copy(void* dst, const void* src, int64_t bytes);
 int asm_device_alloc(void** p, int64_t bytes);
 int asm_device_free(void* p);
-/* Synchronous copy of `bytes` from device memory to host memory. */
+/* Synchronous copy of `bytes` from device memory to host memory, and back. */
 int asm_device_to_host(void* host, const void* dev, int64_t bytes);
+int asm_host_to_device(void* dev, const void* host, int64_t bytes);
 
 #ifdef __cplusplus
 }
