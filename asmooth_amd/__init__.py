@@ -149,6 +149,7 @@ def lib() -> C.CDLL:
         "asm_batch_launch": (i32, [vp]),
         "asm_batch_run_sequential": (i32, [vp]),
         "asm_batch_synchronize": (i32, [vp]),
+        "asm_batch_set_particles": (i32, [vp, i32, vp, i64, i64]),
         "asm_batch_get_mesh": (i32, [vp, i32, i32, vp, P(f64)]),
         "asm_batch_get_rk_dens": (i32, [vp, i32, vp, vp]),
     }
@@ -393,6 +394,12 @@ class Mlp:
             self._L.asm_mlp_destroy(self._h)
             self._h = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
         try:
             self.close()
@@ -496,6 +503,13 @@ class Batch:
 
     def synchronize(self) -> None:
         self._check(self._L.asm_batch_synchronize(self._h), "asm_batch_synchronize")
+
+    def set_particles(self, i: int, particles: np.ndarray, np_local: int) -> None:
+        """Replace share i's records in place (same count; the graph stays)."""
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        self._check(self._L.asm_batch_set_particles(self._h, int(i), _ptr(p), len(p), int(np_local)),
+                    "asm_batch_set_particles")
+        self.shares[i]["np_local"] = int(np_local)
 
     def get_mesh(self, i: int, nc: int) -> tuple[np.ndarray, np.ndarray]:
         mesh = np.empty(6 * nc ** 3, np.float32)

@@ -89,6 +89,10 @@ struct asm_context {
     // Set while a batch records its work into a HIP graph: no host reads,
     // buffers already sized by an eager run of the same work.
     bool capturing = false;
+    // In a batch: a device word set when a deposition's tile list needs more
+    // entries than its buffer holds (a replay on denser inputs than the
+    // eager run that sized it); null outside batches.
+    uint32_t* overflow_flag = nullptr;
     DevBuf<ParticleRec> input;   // caller order
     // sort / tree
     DevBuf<uint32_t> keys, vals, keys_tmp, vals_tmp, hist;
@@ -582,6 +586,7 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
         HIPC(ctx->list.ensure(total > 0 ? total : 1));
     }
     a.list_cap = static_cast<uint32_t>(ctx->list.cap < 0xffffffffull ? ctx->list.cap : 0xffffffffull);
+    if (ctx->overflow_flag) HIPC(launch_flag_overflow(ctx->scan_total.p, a.list_cap, ctx->overflow_flag, s));
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cur.p, ntiles, s));
@@ -1028,6 +1033,7 @@ struct asm_batch {
     };
     std::vector<Cube> cubes;
     DevBuf<double> totals;          // per (cube, nc): sum n, sum halo, sum n^2, sum c (raw)
+    DevBuf<uint32_t> overflow;      // per cube: tile lists overflowed in the last run
     std::vector<size_t> total_off;  // first totals slot of each cube
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1050,6 +1056,7 @@ namespace {
 int run_cube(asm_batch::Cube& c, double* tot)
 {
     asm_context* ctx = c.ctx;
+    if (ctx->overflow_flag) HIPC(launch_fill_zero_u32(ctx->overflow_flag, 1, ctx->stream));
     int rc = asm_tree_construct(ctx, c.left, c.right, nullptr);
     if (!rc) rc = asm_neighbor_search(ctx, kSlots - 1);
     for (size_t i = 0; i < c.ncs.size() && !rc; ++i) {
@@ -1064,6 +1071,22 @@ int run_cube(asm_batch::Cube& c, double* tot)
         HIPC(launch_normalize(m.mesh.p, nc, vol, ctx->stream));
     }
     return rc;
+}
+
+// After the batch's work has finished: ASM_ERR_CAPACITY when a share's
+// tile lists overflowed in the last launch (its meshes are then incomplete).
+int batch_check_overflow(asm_batch* b)
+{
+    if (!b->overflow.p || b->cubes.empty()) return ASM_OK;
+    std::vector<uint32_t> f(b->cubes.size());
+    HIPC2(b, hipMemcpy(f.data(), b->overflow.p, sizeof(uint32_t) * f.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < f.size(); ++i)
+        if (f[i])
+            return batch_fail(b, ASM_ERR_CAPACITY,
+                              "share %zu: the inputs need more tile-list entries than the graph was recorded with; "
+                              "its meshes are incomplete (capture again after changing the inputs this way)",
+                              i);
+    return ASM_OK;
 }
 
 int batch_release_graph(asm_batch* b)
@@ -1196,6 +1219,8 @@ int asm_batch_capture(asm_batch* b, int n_streams)
         slots += 4 * c.ncs.size();
     }
     HIPC2(b, b->totals.ensure(slots));
+    HIPC2(b, b->overflow.ensure(b->cubes.size()));
+    for (size_t i = 0; i < b->cubes.size(); ++i) b->cubes[i].ctx->overflow_flag = b->overflow.p + i;
     // Eager run: every buffer reaches its size.
     for (size_t i = 0; i < b->cubes.size(); ++i) {
         asm_batch::Cube& c = b->cubes[i];
@@ -1297,6 +1322,26 @@ int asm_batch_synchronize(asm_batch* b)
     if (!b) return ASM_ERR_ARG;
     HIPC2(b, hipSetDevice(b->device));
     HIPC2(b, hipStreamSynchronize(b->main));
+    return batch_check_overflow(b);
+}
+
+// Share i's particles replaced in place: same count, so its buffers and the
+// recorded graph stay valid and the next launch smooths the new records.
+int asm_batch_set_particles(asm_batch* b, int i, const asm_particle* particles, int64_t n, int64_t np_local)
+{
+    if (!b || !particles || i < 0 || i >= static_cast<int>(b->cubes.size()))
+        return batch_fail(b, ASM_ERR_ARG, "no share %d", i);
+    asm_batch::Cube& c = b->cubes[i];
+    if (n != c.n) return batch_fail(b, ASM_ERR_ARG, "share %d holds %lld particles, not %lld", i,
+                                    static_cast<long long>(c.n), static_cast<long long>(n));
+    HIPC2(b, hipSetDevice(b->device));
+    HIPC2(b, hipStreamSynchronize(b->main));   // no replay may be reading the old records
+    const int rc = asm_set_particles(c.ctx, particles, n, np_local, c.ctx->boxsize);
+    if (rc) {
+        b->err = asm_last_error(c.ctx);
+        return rc;
+    }
+    c.np_local = np_local;
     return ASM_OK;
 }
 
@@ -1316,6 +1361,8 @@ int asm_batch_get_mesh(asm_batch* b, int i, int nc, float* mesh, double totals[4
     if (k == c.ncs.size()) return batch_fail(b, ASM_ERR_ARG, "share %d has no mesh nc %d", i, nc);
     HIPC2(b, hipSetDevice(b->device));
     HIPC2(b, hipStreamSynchronize(b->main));
+    const int orc = batch_check_overflow(b);
+    if (orc) return orc;
     if (mesh) {
         const int rc = asm_get_mesh(c.ctx, nc, mesh);
         if (rc) {

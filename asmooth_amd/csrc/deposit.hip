@@ -738,6 +738,13 @@ __global__ __launch_bounds__(kKahanThreads) void deposit_tile_kahan_kernel(
     flush_tile(a, sum, cx0, cy0, cz0, mesh, accumulate);
 }
 
+// Sets *flag when the list total counted on the device exceeds the list's
+// capacity (a recorded graph replayed on denser inputs than it was sized for).
+__global__ void flag_overflow_kernel(const uint64_t* __restrict__ total, uint64_t cap, uint32_t* __restrict__ flag)
+{
+    if (threadIdx.x == 0 && *total > cap) *flag = 1u;
+}
+
 template <typename T>
 __global__ void fill_zero_kernel(T* __restrict__ p, int64_t n)
 {
@@ -905,6 +912,12 @@ hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start
     else
         deposit_tile_packed_kernel<double><<<static_cast<unsigned>(ntiles), kPackThreads, 0, s>>>(
             a, tile_start, tile_cnt, list, mesh, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_overflow(const uint64_t* total, uint64_t cap, uint32_t* flag, hipStream_t s)
+{
+    flag_overflow_kernel<<<1, 64, 0, s>>>(total, cap, flag);
     return hipGetLastError();
 }
 

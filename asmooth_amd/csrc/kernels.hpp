@@ -156,6 +156,8 @@ hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start,
                                uint32_t* list, hipStream_t s);
 hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start, const uint32_t* tile_cnt,
                                 const uint32_t* list, float* mesh, int accumulate, hipStream_t s);
+// *flag = 1 when *total > cap (both on the device).
+hipError_t launch_flag_overflow(const uint64_t* total, uint64_t cap, uint32_t* flag, hipStream_t s);
 hipError_t launch_fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s);
 hipError_t launch_fill_zero_f32(float* p, int64_t n, hipStream_t s);
 hipError_t launch_fill_zero_f64(double* p, int64_t n, hipStream_t s);

@@ -296,7 +296,15 @@ int asm_batch_autotune(asm_batch* b, const int* leaf_sizes, int n_leaf, int* lea
 /* Runs every share once (sizing its buffers) and records the graph. */
 int asm_batch_capture(asm_batch* b, int n_streams);
 int asm_batch_launch(asm_batch* b);
+/* Waits for the last launch.  Returns ASM_ERR_CAPACITY when a share's inputs
+ * (changed by asm_batch_set_particles after the capture) needed more tile-list
+ * entries than the recorded graph holds: that share's meshes are incomplete,
+ * and asm_batch_get_mesh fails the same way until a launch fits (capture
+ * again to size the buffers for the new inputs). */
 int asm_batch_synchronize(asm_batch* b);
+/* Share i's particles replaced in place (n must equal the share's count): the
+ * graph stays recorded and the next launch smooths the new records. */
+int asm_batch_set_particles(asm_batch* b, int i, const asm_particle* particles, int64_t n, int64_t np_local);
 /* The same work without the graph, one share after another (a baseline). */
 int asm_batch_run_sequential(asm_batch* b);
 /* Normalised mesh (6 nc^3 floats) and raw FP64 totals {sum n, sum halo,

@@ -49,3 +49,58 @@ def test_batch_graph_matches_single_path(ctx):
             gpu_path(ctx, shares[0][0], shares[0][1], L, bf, ncs, keep_neighbors=False)
     finally:
         b.close()
+
+
+def _records(pos, vel):
+    p = np.zeros(len(pos), A.PARTICLE_DTYPE)
+    p["x"] = pos
+    p["v"] = vel
+    return p
+
+
+def test_batch_replay_on_denser_inputs_reports_capacity(ctx):
+    """A share whose inputs change in place after the capture so that its
+    tile lists need more entries than the graph was recorded with: the
+    synchronize after the replay returns ASM_ERR_CAPACITY (and so does reading
+    its mesh) instead of an incomplete mesh; inputs of the recorded shape
+    replay fine again, and a new capture sizes the buffers for the new ones."""
+    rng = np.random.default_rng(5)
+    L, n, nc = 32.0, 4096, 16
+    # recorded: every particle in a cube of side 4 (h under one cell: about
+    # one tile each); replayed: the same count spread over the whole cube
+    # (h about 4: several tiles each)
+    tight = _records(rng.uniform(14.0, 18.0, (n, 3)).astype(np.float32), rng.normal(0, 1, (n, 3)).astype(np.float32))
+    spread = _records(rng.uniform(0.0, L, (n, 3)).astype(np.float32), rng.normal(0, 1, (n, 3)).astype(np.float32))
+    b = A.Batch(0)
+    try:
+        b.add(tight, n, L, 0.0, L, [nc], -1)
+        b.set_option(A.OPT_TILE, 4)
+        b.capture(n_streams=1)
+        b.launch()
+        b.synchronize()
+        m_tight, _ = b.get_mesh(0, nc)
+        b.set_particles(0, spread, n)
+        b.launch()
+        with pytest.raises(A.AsmoothError, match="status -4"):
+            b.synchronize()
+        with pytest.raises(A.AsmoothError, match="status -4"):
+            b.get_mesh(0, nc)
+        # back to the recorded inputs: the replay fits and gives the same mesh
+        b.set_particles(0, tight, n)
+        b.launch()
+        b.synchronize()
+        m2, _ = b.get_mesh(0, nc)
+        assert np.array_equal(m2.view(np.int32), m_tight.view(np.int32)) or rel_err(m2, m_tight, 1e-30) <= 1e-6
+        # a new capture sizes the lists for the spread inputs, whose mesh then
+        # equals the single-context path's
+        b.set_particles(0, spread, n)
+        b.capture(n_streams=1)
+        b.launch()
+        b.synchronize()
+        m3, _ = b.get_mesh(0, nc)
+        ctx.set_option(A.OPT_TILE, 4)
+        ref = gpu_path(ctx, spread, n, L, 0.0, [nc], keep_neighbors=False, tile=4)
+        raw = ref["meshes"][nc]
+        assert rel_err(m3[: nc ** 3], raw[: nc ** 3] * np.float32(8), floor=1e-30) <= 1e-6
+    finally:
+        b.close()

@@ -1,0 +1,234 @@
+"""The larger workloads as parity tests on the GPU.
+
+- 64 batched node shares (cubes of side 128, 2^15-2^21 particles each,
+  Zel'dovich rms 1-4, buffer 6 units, nc 32 / 64 / 128) through one recorded
+  HIP graph: every share's FP64 mass totals, the smallest share against the
+  oracle whole, the largest on sampled regions.
+- 512^3 Zel'dovich particles in a box of side 1024 plus 5000 Gaussian blobs
+  holding 10% of them, one node share, nc 512: h, density and mesh cells on
+  sampled regions against the oracle; the reference's mass check; the FP32
+  Kahan and FP32 atomic cell sums within their bounds of the same oracle
+  cells.
+- The held-out 512^3 set (seed 200), one node share of a 2^3 split, with the
+  trained seed model: seeded h bit-exact against the plain search over the
+  whole share, the fallback rate below its bound, and the seeded h tied to
+  the oracle on a sampled region.
+
+Sets too large for the oracle whole are checked by region_oracle
+(tests/helpers.py), whose own assertions prove that the sampled particles'
+neighbour spheres lie inside the region it searches.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import asmooth_amd as A
+from asmooth_amd import synth
+import oracle as O
+from tests.helpers import mesh_at, oracle_path, region_oracle, rel_err
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+RTOL = 1e-6
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _as_buf(parts):
+    return np.concatenate([parts["x"], parts["v"]], axis=1).astype(np.float32)
+
+
+def _subset_region(buf, L, center, half, hmax, ncs):
+    """region_oracle on the particles near `center` only (the full set of a
+    512^3 run is too large to scan in float64 per region): the subset holds
+    every particle within the cube region_oracle searches, so its results are
+    those of the whole set; indices are mapped back to `buf`."""
+    m1 = hmax + max(L / nc for nc in ncs) + 1.0
+    m2 = m1 + 2.0 * hmax + 1.0
+    c = np.asarray(center, np.float32)
+    reach = np.float32(half + m2 + 1.0)
+    sel = np.nonzero(np.all(np.abs(buf[:, :3] - c) < reach, axis=1))[0]
+    r = region_oracle(buf[sel], L, center, half, hmax, ncs)
+    r["index"] = sel[r["index"]]
+    r["nbr"] = sel[r["nbr"]]
+    return r
+
+
+def _check_region(r, rk, dens, mesh, nc, vmax, rtol_cells=RTOL):
+    idx = r["index"]
+    assert len(idx) > 100
+    assert np.array_equal(rk[idx].view(np.int32), r["rk"].view(np.int32))
+    assert rel_err(dens[idx], r["dens64"], floor=1e-30) <= RTOL
+    cells, fields = r["cells"][nc]
+    assert len(cells) > 0
+    g = mesh_at(mesh, nc, cells)
+    for f in range(3):   # n, halo, c
+        assert rel_err(g[f], fields[f], floor=1e-30) <= rtol_cells, f
+    scale = fields[0] * vmax + 1e-30
+    for f in range(3, 6):
+        assert np.max(np.abs(g[f] - fields[f]) / scale) <= rtol_cells, f
+
+
+def _hmax_near(buf, rk, center, reach):
+    near = np.all(np.abs(buf[:, :3] - np.float32(center)) < np.float32(reach), axis=1)
+    return float(rk[near].max()) * 1.001
+
+
+# ----------------------------------------------------------- 64 shares ---
+
+def test_batched_64_shares_match_oracle():
+    L, bf, ncs = 128.0, 6.0 / 128.0, [32, 64, 128]
+    cubes = synth.batched_cubes(n_cubes=64, L=L, seed0=1000)
+    shares = [A.build_buffered_set(xv, L, bf) for xv in cubes]
+    del cubes
+    _log(f"batched: 64 shares, {sum(len(p) for p, _ in shares)} buffered particles")
+    b = A.Batch(0)
+    try:
+        for p, nl in shares:
+            b.add(p, nl, L, -bf * L, (1 + bf) * L, ncs, -1)
+        b.autotune()
+        b.capture(n_streams=8)
+        b.launch()
+        b.synchronize()
+        # FP64 mass totals of every share and mesh (check_total_mass,
+        # clumping_tree_main.cpp:578-600).  Shares whose largest h exceeds
+        # the 6-unit buffer keep buffer particles' weight inside the node
+        # cube, as in the reference, so their totals exceed np_local a little.
+        for i, (p, nl) in enumerate(shares):
+            rk, _ = b.get_rk_dens(i)
+            fits = float(rk.max()) < 6.0
+            for nc in ncs:
+                _, tot = b.get_mesh(i, nc)
+                ratio = (tot[0] + tot[1]) / nl
+                if fits:
+                    assert 0.999 < ratio < 1.001, (i, nc, ratio)
+                else:
+                    assert 0.999 < ratio < 1.01, (i, nc, ratio)
+        sizes = [len(p) for p, _ in shares]
+        small, large = int(np.argmin(sizes)), int(np.argmax(sizes))
+        # the smallest share against the oracle, whole
+        p, nl = shares[small]
+        buf = _as_buf(p)
+        o = oracle_path(buf, L, ncs)
+        rk, dens = b.get_rk_dens(small)
+        assert np.array_equal(rk.view(np.int32), o["rk"].view(np.int32))
+        assert rel_err(dens, o["dens64"], floor=1e-30) <= RTOL
+        vmax = float(np.max(np.abs(buf[:, 3:])))
+        for nc in ncs:
+            mesh, _ = b.get_mesh(small, nc)            # normalised
+            want = O.normalize_f64(o["meshes"][nc], nc, L)
+            m = nc ** 3
+            assert rel_err(mesh[:m], want[:m], floor=1e-30) <= RTOL          # n = 8 * sum
+            occ = o["meshes"][nc][:m] > 1e-3
+            # c * vol / n^2 and v / n carry the relative errors of n, c, v
+            assert rel_err(mesh[2 * m:3 * m][occ], want[2 * m:3 * m][occ], floor=1e-30) <= 4 * RTOL
+            v_err = np.abs(mesh[3 * m:].reshape(m, 3)[occ] - want[3 * m:].reshape(m, 3)[occ]) / vmax
+            assert float(v_err.max()) <= 4 * RTOL
+        # the largest share on two sampled regions (one at a node corner)
+        p, nl = shares[large]
+        buf = _as_buf(p)
+        rk, dens = b.get_rk_dens(large)
+        vmax = float(np.max(np.abs(buf[:, 3:])))
+        for center in ((0.0, 0.0, 0.0), (70.0, 40.0, 95.0)):
+            hmax = _hmax_near(buf, rk, center, 60.0)
+            r = _subset_region(buf, L, center, 8.0, hmax, [ncs[-1]])
+            mesh, _ = b.get_mesh(large, ncs[-1])
+            m = ncs[-1] ** 3
+            idx = r["index"]
+            assert np.array_equal(rk[idx].view(np.int32), r["rk"].view(np.int32))
+            assert rel_err(dens[idx], r["dens64"], floor=1e-30) <= RTOL
+            cells, fields = r["cells"][ncs[-1]]
+            assert rel_err(mesh[cells], np.float32(8) * fields[0], floor=1e-30) <= RTOL
+    finally:
+        b.close()
+
+
+# ------------------------------------------------- 512^3 + 5000 blobs ---
+
+def test_precision_set_512_nc512_regions_and_fp32_modes():
+    n, L, bf, nc = 512, 1024.0, 0.05, 512
+    xv = synth.precision_study(n=n, L=L, seed=2, n_blobs=5000)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    del xv
+    _log(f"precision set: {len(parts)} buffered / {nl} local")
+    buf = _as_buf(parts)
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    with A.Context(0) as c:
+        c.set_particles(parts, nl, L)
+        del parts
+        c.construct(-bf * L, (1 + bf) * L)
+        c.neighbor_search(32)
+        rk, dens, _ = c.get_rk_dens()
+        c.assign_on_mesh(nc, -1)
+        tot = c.mesh_total(nc)
+        # the reference's mass check (clumping_tree_main.cpp:597-598)
+        assert 0.999 < (tot[0] + tot[1]) / nl < 1.001
+        mesh = c.get_mesh(nc)
+        m = nc ** 3
+        # regions: the node corner (clipped cell ranges, buffer particles)
+        # and the densest cell of the mesh (a blob)
+        dense = int(np.argmax(mesh[:m]))
+        dx = L / nc
+        dense_c = ((dense % nc + 0.5) * dx, ((dense // nc) % nc + 0.5) * dx, (dense // nc ** 2 + 0.5) * dx)
+        regions = []
+        for center in ((0.0, 0.0, 0.0), dense_c):
+            hmax = _hmax_near(buf, rk, center, 60.0)
+            r = _subset_region(buf, L, center, 6.0, hmax, [nc])
+            _check_region(r, rk, dens, mesh, nc, vmax)
+            regions.append(r)
+        # FP32 cell sums against the same oracle cells: Kahan-compensated
+        # within the 1e-6 bar, plain FP32 atomics (the reference's own
+        # scheme, coarse_mesh.cpp:95-121) within 1e-5
+        for mode, bar in ((1, 1e-6), (2, 1e-5)):
+            c.set_option(A.OPT_DEPOSIT_ACCUM, mode)
+            c.assign_on_mesh(nc, -1)
+            m32 = c.get_mesh(nc)
+            for r in regions:
+                cells, fields = r["cells"][nc]
+                g = mesh_at(m32, nc, cells)
+                assert rel_err(g[0], fields[0], floor=1e-30) <= bar, (mode, rel_err(g[0], fields[0], floor=1e-30))
+                assert rel_err(g[2], fields[2], floor=1e-30) <= bar, mode
+        c.set_option(A.OPT_DEPOSIT_ACCUM, 0)
+
+
+# ------------------------------------------- held-out learned-seed set ---
+
+def test_learned_seed_heldout_share_is_exact():
+    from asmooth_amd.learned_seed import DEFAULT_MODEL
+    assert os.path.exists(DEFAULT_MODEL)
+    n, box, nn, bf = 512, 1024.0, 2, 0.05
+    xv = synth.zeldovich(n=n, L=box, rms=2.5, seed=200)
+    parts, nl = A.build_node_share(xv, box, nn, (1, 0, 1), bf)
+    del xv
+    L = box / nn
+    _log(f"held-out share: {len(parts)} buffered / {nl} local")
+    with A.Context(0) as c, A.Mlp(0, np.load(DEFAULT_MODEL)) as model:
+        c.set_particles(parts, nl, L)
+        c.construct(-bf * L, (1 + bf) * L)
+        c.neighbor_search(32)
+        rk, dens, _ = c.get_rk_dens()
+        plain_work = c.search_stats()
+        c.set_seed_model(model, 0.3)
+        c.construct(-bf * L, (1 + bf) * L)
+        c.neighbor_search(32)
+        srk, sdens, _ = c.get_rk_dens()
+        st = c.seed_stats()
+        seeded_work = c.search_stats()
+        c.set_seed_model(None)
+    assert np.array_equal(srk.view(np.int32), rk.view(np.int32))
+    assert rel_err(sdens, dens, floor=1e-30) <= RTOL
+    rate = st["fallback_particles"] / len(parts)
+    _log(f"fallback rate {rate:.2e}; distance evaluations per query plain "
+         f"{plain_work['dist_evals'] / len(parts):.1f}, seeded {seeded_work['dist_evals'] / len(parts):.1f}")
+    assert rate < 0.01
+    # the model bounds the search: it never makes the bound walk evaluate more
+    assert seeded_work["dist_evals"] <= plain_work["dist_evals"] * 1.01
+    # the seeded result is the oracle's on a sampled region
+    buf = _as_buf(parts)
+    center = (100.0, 300.0, 420.0)
+    r = _subset_region(buf, L, center, 6.0, _hmax_near(buf, srk, center, 60.0), [64])
+    assert np.array_equal(srk[r["index"]].view(np.int32), r["rk"].view(np.int32))
+    assert rel_err(sdens[r["index"]], r["dens64"], floor=1e-30) <= RTOL
