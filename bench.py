@@ -201,6 +201,8 @@ def cpu_baseline(xv: np.ndarray, L: float, bf: float, ncs, nn: int = 2):
     share, nl = A.build_node_share(xv, L, nn, (0, 0, 0), bf)
     Ln = L / nn
     sub_ncs = [nc // nn for nc in ncs]
+    # OpenMP threads: the host's cores, at most 16 (the CPU share of one GPU
+    # on the GPU hosts; host_cpus records what the OS reports).
     cores = min(os.cpu_count() or 1, 16)
     pos = np.ascontiguousarray(share["x"])
     vel = np.ascontiguousarray(share["v"])
@@ -210,6 +212,25 @@ def cpu_baseline(xv: np.ndarray, L: float, bf: float, ncs, nn: int = 2):
     T = float(sec.sum())
     log(f"[bench] cpu reference: construction {sec[0]:.3f}s neighbors {sec[1]:.3f}s "
         f"mesh_assign {sec[2]:.3f}s ({cores} threads, wall {wall:.1f}s)")
+    # The reference's own h and neighbour lists on the same share against the
+    # device path's (untimed): particles whose h, 33-slot set, 32-nearest set
+    # or slot order differ (its d2 < fl(sqrt(K))^2 filter, nbr_search.cpp:94,103,
+    # and its discovery-order ties, k_neighbors.h:26-28).
+    R = O.Ref(pos, vel, boxsize=Ln, np_local=nl)
+    try:
+        R.construct(-bf * Ln, (1.0 + bf) * Ln)
+        ref_nbr, _, ref_rk = R.knn_lists(cores)
+    finally:
+        R.close()
+    with A.Context(0) as c:
+        c.set_option(A.OPT_KEEP_NEIGHBORS, 1)
+        c.set_particles(share, nl, Ln)
+        c.construct(-bf * Ln, (1.0 + bf) * Ln)
+        c.neighbor_search(32)
+        g_rk, _, _ = c.get_rk_dens()
+        g_nbr = c.get_neighbors()
+    vs_ref = O.compare_with_reference(g_nbr, g_rk, ref_nbr, ref_rk)
+    log(f"[bench] device vs reference on the same share: {vs_ref}")
     return {
         "value": nl / T, "unit": "particles/s", "cores": cores, "kind": "reference",
         "sample": (f"node (0,0,0) of a {nn}^3 split of the same 256^3 set: node cube {Ln:g}, "
@@ -217,6 +238,8 @@ def cpu_baseline(xv: np.ndarray, L: float, bf: float, ncs, nn: int = 2):
                    f"(same cell sizes); reference construct+neighbor loop+assign_on_mesh "
                    f"(1 rank, {cores} OpenMP threads)"),
         "seconds": {"construction": float(sec[0]), "neighbors": float(sec[1]), "mesh_assign": float(sec[2])},
+        "host_cpus": os.cpu_count(),
+        "device_vs_reference": vs_ref,
     }
 
 

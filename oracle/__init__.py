@@ -84,7 +84,7 @@ def rlib() -> C.CDLL:
         L.ref_neighbor_search_all.restype = i32
         L.ref_neighbor_search_all.argtypes = [vp, i32, P(f64)]
         L.ref_knn_lists.restype = i32
-        L.ref_knn_lists.argtypes = [vp, vp, vp, vp]
+        L.ref_knn_lists.argtypes = [vp, vp, vp, vp, i32]
         L.ref_get_particles.restype = i32
         L.ref_get_particles.argtypes = [vp, vp, vp, vp]
         L.ref_set_dens.restype = i32
@@ -235,11 +235,11 @@ class Ref:
         self.L.ref_neighbor_search_all(self.h, int(nthreads), C.byref(t))
         return t.value
 
-    def knn_lists(self):
+    def knn_lists(self, nthreads: int = 1):
         nbr = np.empty((self.n, K_SLOTS), np.int32)
         d2 = np.empty((self.n, K_SLOTS), np.float32)
         rk = np.empty(self.n, np.float32)
-        self.L.ref_knn_lists(self.h, _p(nbr), _p(d2), _p(rk))
+        self.L.ref_knn_lists(self.h, _p(nbr), _p(d2), _p(rk), int(nthreads))
         return nbr, d2, rk
 
     def particles(self):
@@ -272,6 +272,24 @@ class Ref:
 def ref_write_mesh_separate(base: str, inode: int, mesh: np.ndarray, nc: int, L: float) -> None:
     m = np.ascontiguousarray(mesh, np.float32).copy()
     rlib().ref_write_mesh_separate(base.encode(), int(inode), _p(m), int(nc), float(L))
+
+
+def compare_with_reference(nbr, rk, ref_nbr, ref_rk):
+    """Particles whose result differs from the reference's own search: h
+    (bit pattern), the 33-slot neighbour set, the set of the 32 nearest
+    (the particle itself and the 31 its density term goes to,
+    nbr_search.cpp:78-84) and the order of the 33 slots (the reference ranks
+    equal distances by discovery, k_neighbors.h:26-28; the canonical order
+    ranks them by index)."""
+    nbr = np.asarray(nbr)
+    ref_nbr = np.asarray(ref_nbr)
+    return {
+        "h": int(np.count_nonzero(np.asarray(rk).view(np.int32) != np.asarray(ref_rk).view(np.int32))),
+        "set33": int(np.count_nonzero(np.any(np.sort(nbr, 1) != np.sort(ref_nbr, 1), axis=1))),
+        "set32": int(np.count_nonzero(np.any(np.sort(nbr[:, :32], 1) != np.sort(ref_nbr[:, :32], 1), axis=1))),
+        "order": int(np.count_nonzero(np.any(nbr != ref_nbr, axis=1))),
+        "particles": int(len(nbr)),
+    }
 
 
 def ref_time_path(pos, vel, np_local: int, L: float, bf: float, ncs, nthreads: int):

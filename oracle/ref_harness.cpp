@@ -137,14 +137,18 @@ int ref_neighbor_search_all(void* h, int nthreads, double* seconds)
 
 // The reference's own neighbour list for every particle (find_nbr_recursive),
 // returned ascending (slot 0 = nearest) in original indices.  rk is sqrtf of
-// the farthest slot as in nbr_search.cpp:73.
-int ref_knn_lists(void* h, int32_t* nbr, float* d2, float* rk)
+// the farthest slot as in nbr_search.cpp:73.  Each particle's search is
+// independent (the loop of clumping_tree_main.cpp:425-430 without its density
+// scatter), so nthreads > 1 gives the same lists.
+int ref_knn_lists(void* h, int32_t* nbr, float* d2, float* rk, int nthreads)
 {
     RefState* s = (RefState*)h;
+    set_threads(nthreads);
     set_nbr_search(s->tree->root(), s->particles->particle);
     Particle* const p = s->particles->particle;
     const int k = KNeighbors::k;
     KNeighbors knbrs;
+#pragma omp parallel for private(knbrs) schedule(dynamic, 4096)
     for (int i = 0; i < s->n; ++i) {
         knbrs.clear();
         find_nbr_recursive(s->tree->root(), p[i].x, &knbrs);

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import asmooth_amd as A
+from asmooth_amd import synth
 import oracle as O
 from tests import helpers
 from tests.helpers import mesh_fields, small_set
@@ -334,3 +335,30 @@ def test_region_oracle_equals_whole_set_oracle(centre):
         assert len(cells) > 0
         whole = O.assign_on_mesh(pos, vel, dens32, rk, np.zeros(len(b), np.int32), L, nc, -1)
         assert np.array_equal(o6, helpers.mesh_at(whole, nc, cells))
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (reference tree absent)")
+def test_reference_disagreements_on_the_64cubed_blob_set():
+    """The 64^3 blob set whole (349 k buffered particles): the reference's own
+    search (kdtree_balanced.cpp + nbr_search.cpp, compiled here) against the
+    oracle's canonical kNN.  Its d2 < fl(sqrt(K))^2 filter
+    (nbr_search.cpp:94,103) changes no h and no neighbour set here; only the
+    order of equal distances differs (discovery order, k_neighbors.h:26-28),
+    and every such row holds the same distances slot by slot."""
+    L, bf = 128.0, 0.05
+    xv = synth.blobs_uniform(n_side=64, L=L, seed=0)
+    buf, nl = O.buffered_set(xv, L, bf)
+    pos, vel = buf[:, :3], buf[:, 3:]
+    onbr, od2, ork = O.knn(pos)
+    R = O.Ref(pos, vel, boxsize=L, np_local=nl)
+    try:
+        R.construct(-bf * L, (1 + bf) * L)
+        rnbr, rd2, rrk = R.knn_lists(nthreads=0)
+    finally:
+        R.close()
+    d = O.compare_with_reference(onbr, ork, rnbr, rrk)
+    assert d["particles"] == len(pos)
+    assert d["h"] == 0 and d["set33"] == 0 and d["set32"] == 0, d
+    rows = np.nonzero(np.any(onbr != rnbr, axis=1))[0]
+    assert len(rows) == d["order"] and d["order"] < 100
+    assert np.array_equal(od2[rows].view(np.int32), rd2[rows].view(np.int32))
