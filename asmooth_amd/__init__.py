@@ -49,6 +49,9 @@ OPT_DETERMINISTIC = 5
 OPT_KNN_METHOD = 6
 OPT_DEPOSIT_ACCUM = 7
 OPT_SEED_WALK = 8
+OPT_KERNEL = 9
+KERNEL_TENT = 0
+KERNEL_SPLINE = 1
 # Limits of asm_smooth_noalloc (include/asmooth.h).
 NOALLOC_MAX_PARTICLES = 1 << 19
 NOALLOC_MAX_NC = 128

@@ -110,6 +110,7 @@ struct asm_context {
     int deterministic = 0;
     int knn_method = kKnnBoundCollect;
     int deposit_accum = kAccumFp64;
+    int kernel = kKernTent;   // ASM_OPT_KERNEL
     DevBuf<uint32_t> knn_flags, knn_leaf_n, knn_leaves;   // per wavefront of 64 sorted particles
     DevBuf<float> knn_tau;                                // per sorted particle
     DevBuf<double> dens_pull;
@@ -333,6 +334,12 @@ int asm_set_option(asm_context* ctx, int option, int value)
             return fail(ctx, ASM_ERR_ARG, "deposition accumulation %d is not 0, 1 or 2", value);
         ctx->deposit_accum = value;
         return ASM_OK;
+    case ASM_OPT_KERNEL:
+        if (value != kKernTent && value != kKernSpline)
+            return fail(ctx, ASM_ERR_ARG, "kernel %d is not 0 (tent) or 1 (cubic spline)", value);
+        ctx->kernel = value;
+        ctx->have_search = false;   // densities depend on the kernel: search again
+        return ASM_OK;
     case ASM_OPT_TILE:
         if (value < 1 || value > 8) return fail(ctx, ASM_ERR_ARG, "tile %d outside 1..8", value);
         ctx->tile = value;
@@ -445,7 +452,8 @@ int asm_neighbor_search(asm_context* ctx, int k)
         if (!ctx->deterministic) HIPC(ctx->dens_pull.ensure(n));
     }
     KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p,
-                    nullptr, nullptr, nullptr, ctx->leaf_size % kGroup == 0 ? ctx->groups.p : nullptr};
+                    nullptr, nullptr, nullptr, ctx->leaf_size % kGroup == 0 ? ctx->groups.p : nullptr, 0,
+                    ctx->kernel};
     ctx->seed_used = false;
     if (bound_collect && ctx->seed_on) {
         HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));
@@ -487,7 +495,7 @@ int asm_neighbor_search(asm_context* ctx, int k)
         if (ctx->deterministic) {
             HIPC(launch_kmax(ctx->kkey.p, n, ctx->leaf_size, ctx->nleaf, ctx->nleaf_pad, ctx->kmax.p, s));
             HIPC(launch_dens_gather(ctx->spos.p, ctx->kkey.p, ctx->rk.p, n, ctx->nodes.p, ctx->kmax.p,
-                                    ctx->nleaf_pad, ctx->dens.p, s));
+                                    ctx->nleaf_pad, ctx->dens.p, ctx->kernel, s));
         } else {
             HIPC(launch_dens_finalize(ctx->dens64.p, bound_collect ? ctx->dens_pull.p : nullptr, n, ctx->dens.p, s));
         }
@@ -557,6 +565,7 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     a.ntx = (nc + a.tile - 1) / a.tile;
     a.periodic = periodic;
     a.accum = ctx->deposit_accum;
+    a.kern = ctx->kernel;
     a.coord[0] = coord ? coord[0] : 0;
     a.coord[1] = coord ? coord[1] : 0;
     a.coord[2] = coord ? coord[2] : 0;

@@ -50,6 +50,42 @@ __device__ __forceinline__ float ker_integ(float x1, float x2)
     return fac2 * tent_f(fac2 * x2) - fac1 * tent_f(fac1 * x1);
 }
 
+// The cubic spline option on the mesh: its separable form, the product over
+// the three axes of the 1-D cubic B-spline of half-width h normalised to one,
+//   w(t) = 4/3 (1 - 6 t^2 + 6 |t|^3) for |t| < 1/2, 8/3 (1 - |t|)^3 for
+//   1/2 <= |t| <= 1,
+// integrated exactly over each cell as the tent is (coarse_mesh.cpp:10-22):
+// G(u) = int_0^u w, G(u) = 4/3 (u - 2 u^3 + 3/2 u^4) below 1/2,
+// 1/2 - 2/3 (1 - u)^4 up to 1, then 1/2; the cell weight is
+// F(x2) - F(x1) with F(t) = sign(t) G(|t|).  The support and so the cell
+// ranges are the tent's.
+__device__ __forceinline__ float spline_g(float u)
+{
+    if (u >= 1.0f) return 0.5f;
+    if (u < 0.5f) {
+        const float u2 = u * u;
+        const float a = 2.0f * (u2 * u);
+        const float b = 1.5f * (u2 * u2);
+        return (4.0f / 3.0f) * ((u - a) + b);
+    }
+    const float r1 = 1.0f - u;
+    const float r2 = r1 * r1;
+    return 0.5f - (2.0f / 3.0f) * (r2 * r2);
+}
+
+__device__ __forceinline__ float spline_integ(float x1, float x2)
+{
+    const float fac1 = (x1 > 0.0f) ? 1.0f : -1.0f;
+    const float fac2 = (x2 > 0.0f) ? 1.0f : -1.0f;
+    return fac2 * spline_g(fac2 * x2) - fac1 * spline_g(fac1 * x1);
+}
+
+// The cell integral of the context's kernel (uniform across a launch).
+__device__ __forceinline__ float cell_integ(int kern, float x1, float x2)
+{
+    return kern == kKernSpline ? spline_integ(x1, x2) : ker_integ(x1, x2);
+}
+
 // Cell range [lo, hi) of one axis.  Node mesh: coarse_mesh.cpp:72-75
 // (truncation toward zero, clipped to [0, nc)).  Periodic variant:
 // global_mesh.cpp:66-69 (floor / ceil, unclipped, wrapped at use).
@@ -407,7 +443,6 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
     const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
     const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
     const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
-    const int ncell = T * T * T;
     for (int c = tid; c < 6 * kTileCells; c += kDepThreads) (&acc[0][0])[c] = 0.0;
     __syncthreads();
 
@@ -459,14 +494,14 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
                     // x weights of this row segment, reused for every (z, y).
                     for (int ix = x0; ix < x1; ++ix) {
                         const float s1 = (static_cast<float>(ix) * dxc - r.x[0]) * hinv;
-                        wxs[tid][ix - x0] = ker_integ(s1, s1 + ds);
+                        wxs[tid][ix - x0] = cell_integ(a.kern, s1, s1 + ds);
                     }
                     for (int iz = z0; iz < z1; ++iz) {
                         const float s1z = (static_cast<float>(iz) * dxc - r.x[2]) * hinv;
-                        const float kzw = ker_integ(s1z, s1z + ds);
+                        const float kzw = cell_integ(a.kern, s1z, s1z + ds);
                         for (int iy = y0; iy < y1; ++iy) {
                             const float s1y = (static_cast<float>(iy) * dxc - r.x[1]) * hinv;
-                            const float kyw = ker_integ(s1y, s1y + ds);
+                            const float kyw = cell_integ(a.kern, s1y, s1y + ds);
                             const int rowc = ((iz - sz - cz0) * T + (iy - sy - cy0)) * T - cx0 - sx;
                             for (int ix = x0; ix < x1; ++ix) {
                                 const float kxw = wxs[tid][ix - x0];
@@ -573,7 +608,7 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
                 n3[d] = hi > lo ? hi - lo : 0;
                 for (int t = 0; t < n3[d]; ++t) {
                     const float s1 = (static_cast<float>(lo + t) * dxc - r.x[d]) * hinv;
-                    W[lane * kWStride + d * kTileMax + t] = ker_integ(s1, s1 + ds);
+                    W[lane * kWStride + d * kTileMax + t] = cell_integ(a.kern, s1, s1 + ds);
                 }
             }
             size = n3[0] * n3[1] * n3[2];
@@ -698,7 +733,7 @@ __global__ __launch_bounds__(kKahanThreads) void deposit_tile_kahan_kernel(
             const int d = lane / kTileMax, t = lane % kTileMax;
             if (t < n3[d]) {
                 const float s1 = (static_cast<float>(c0[d] + b0[d] + t) * dxc - r.x[d]) * hinv;
-                wts[lane] = ker_integ(s1, s1 + ds);
+                wts[lane] = cell_integ(a.kern, s1, s1 + ds);
             }
         }
         __syncthreads();

@@ -45,6 +45,11 @@ hipError_t launch_build_tree(const float4* spos, int64_t n, int leaf_size, int64
                              int64_t nleaf_pad, Box* nodes, Box* groups, hipStream_t s);
 
 // ---- knn.hip ---------------------------------------------------------------
+// Smoothing kernels (ASM_OPT_KERNEL): the reference's separable tent
+// (nbr_search.cpp:27-37, coarse_mesh.cpp:10-22) or the cubic spline
+// (nbr_search.cpp:39-53 for the density; its separable form on the mesh).
+constexpr int kKernTent = 0;
+constexpr int kKernSpline = 1;
 struct KnnStats {
     unsigned long long node_tests;
     unsigned long long leaf_scans;
@@ -87,6 +92,8 @@ struct KnnScratch {
     // With tau_init: 1 runs walk 1 as a range walk within the learned bounds
     // (knn_seed_walk_kernel), 0 as the bounded walk 1.
     int seed_walk;
+    // Density kernel of the scatter: kKernTent or kKernSpline.
+    int kernel;
 };
 // Optional marks around the kernels of the bound/collect method (stage 0:
 // bound walk, 1: collect walk, 2: insertion kernel for flagged wavefronts).
@@ -106,7 +113,7 @@ hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t n
                        hipStream_t s);
 // Density by gather (sorted order, float).
 hipError_t launch_dens_gather(const float4* spos, const uint64_t* kkey, const float* rk, int64_t n, const Box* nodes,
-                              const float* kmax, int64_t nleaf_pad, float* dens, hipStream_t s);
+                              const float* kmax, int64_t nleaf_pad, float* dens, int kern, hipStream_t s);
 
 // ---- mlp.hip -----------------------------------------------------------------
 // Learned seed radius: 27 inputs (ln CIC density over 3x3x3 cells of a
@@ -147,6 +154,7 @@ struct DepositArgs {
     int coord[3];    // node coordinate for the periodic variant
     uint32_t list_cap;   // entries the tile list holds (writes and reads past it are dropped)
     int accum;           // kAccumFp64 (default), kAccumKahan32, kAccumAtomic32 (node mesh only)
+    int kern;            // kKernTent (default) or kKernSpline: the cell integral of the kernel
 };
 constexpr int kAccumFp64 = 0;      // FP64 LDS atomics
 constexpr int kAccumKahan32 = 1;   // FP32 with Kahan compensation, one wavefront per tile

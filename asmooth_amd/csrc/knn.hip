@@ -79,6 +79,33 @@ __device__ __forceinline__ float tent_weight(float x0, float x1, float x2, float
     return w;
 }
 
+// The density kernels: the reference's separable tent (ker,
+// nbr_search.cpp:27-37, the one its neighbour loop uses) and its cubic
+// spline (spline_ker, nbr_search.cpp:39-53, present there but not called),
+// selected per context (kKernTent / kKernSpline).  The spline is radial: it
+// takes the squared distance d2 = norm2(x, y), which is the same bit for bit
+// with x and y swapped.  Its arithmetic follows spline_ker: the factor
+// 8/pi hinv^3 evaluated in FP64 (M_PI is a double) and rounded once to FP32,
+// r = sqrtf(d2) * hinv, then the two polynomial pieces in FP32, left to right.
+__device__ __forceinline__ float spline_weight(float d2, float hinv)
+{
+    const double h = static_cast<double>(hinv);
+    const float fac = static_cast<float>(((8.0 / M_PI) * h) * h * h);
+    const float r = sqrtf(d2) * hinv;
+    if (r < 0.5f) {
+        const float r6 = 6.0f * r;
+        return fac * ((1.0f - r6 * r) + (r6 * r) * r);
+    }
+    const float r1 = 1.0f - r;
+    return fac * (((2.0f * r1) * r1) * r1);
+}
+
+__device__ __forceinline__ float dens_weight(int kern, float x0, float x1, float x2, float y0, float y1, float y2,
+                                             float hinv, float d2)
+{
+    return kern == kKernSpline ? spline_weight(d2, hinv) : tent_weight(x0, x1, x2, y0, y1, y2, hinv);
+}
+
 // The wavefront's index in its block, as a scalar: the compiler cannot tell
 // that threadIdx.x / 64 is uniform, and per-wavefront loop bounds derived
 // from it would otherwise live in VGPRs with divergent loop exits.
@@ -129,7 +156,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
     int64_t nleaf, int leaf_size, int init_leaves,
     float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted, const int32_t* __restrict__ inv_perm,
     double* __restrict__ dens64, int32_t* __restrict__ nbr_out, KnnStats* __restrict__ stats,
-    const uint32_t* __restrict__ only_flagged)
+    const uint32_t* __restrict__ only_flagged, int kern)
 {
     __shared__ float4 stage[kWavesPerBlock][kWave];
     const int lane = threadIdx.x % kWave;
@@ -227,7 +254,8 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
                 if (ties_only && key_d2(key[s]) < kd) continue;
                 const int32_t sq = inv_perm[key_idx(key[s])];
                 const float4 c = spos[sq];
-                unsafeAtomicAdd(&dens64[sq], static_cast<double>(tent_weight(qx, qy, qz, c.x, c.y, c.z, hinv)));
+                unsafeAtomicAdd(&dens64[sq], static_cast<double>(dens_weight(kern, qx, qy, qz, c.x, c.y, c.z, hinv,
+                                                                             key_d2(key[s]))));
             }
         }
         if (nbr_out) {
@@ -595,7 +623,7 @@ struct CollectLds {
 template <int M, int P, bool kAll>
 __device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int count, uint64_t mask, float4 c,
                                              float tc, int lane, float qx, float qy, float qz, float tau, float hinv,
-                                             double* __restrict__ dens64, unsigned long long& n_pairs)
+                                             double* __restrict__ dens64, unsigned long long& n_pairs, int kern)
 {
     constexpr int G = kWave / P;
     const int j = lane % P;
@@ -620,15 +648,15 @@ __device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int co
             if (kAll && slot < static_cast<uint32_t>(M)) L.list[own][slot] = static_cast<uint32_t>(start + j);
             if (d2 < o.w) {
                 if (dens64 && d2 > tc)
-                    unsafeAtomicAdd(&dens64[start + j],
-                                    static_cast<double>(tent_weight(o.x, o.y, o.z, c.x, c.y, c.z, L.htab[k])));
+                    unsafeAtomicAdd(&dens64[start + j], static_cast<double>(dens_weight(
+                                                            kern, o.x, o.y, o.z, c.x, c.y, c.z, L.htab[k], d2)));
             } else {
                 const uint32_t e = atomicAdd(&L.ecnt[own], 1u);
                 L.tidx[own] = __float_as_uint(c.w);
                 if (!kAll && e < static_cast<uint32_t>(M)) L.list[own][e] = static_cast<uint32_t>(start + j);
             }
             if (dens64 && d2 < tc)
-                atomicAdd(&L.pacc[own], static_cast<double>(tent_weight(c.x, c.y, c.z, o.x, o.y, o.z, hc)));
+                atomicAdd(&L.pacc[own], static_cast<double>(dens_weight(kern, c.x, c.y, c.z, o.x, o.y, o.z, hc, d2)));
         }
     }
     wave_lds_fence();
@@ -655,7 +683,7 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
     int leaf_size, int init_leaves, const uint32_t* __restrict__ wleaf, const uint32_t* __restrict__ wleaf_n,
     const float* __restrict__ tau_sorted, float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted,
     double* __restrict__ dens64, double* __restrict__ dens_pull, int32_t* __restrict__ nbr_out,
-    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats)
+    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats, int kern)
 {
     __shared__ CollectLds<M> lds[kCollectWaves];
     const int lane = threadIdx.x % kWave;
@@ -736,7 +764,7 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
                                       static_cast<uint32_t>(__builtin_amdgcn_readlane(m_lo, cur));
                 ++n_leaf;
                 collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64,
-                                         n_pairs);
+                                         n_pairs, kern);
             }
         }
     } else {
@@ -755,7 +783,8 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
                 const int count = b.hi_w;
                 const float4 c = (j < count) ? spos[start + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
                 const float tc = (j < count) ? tau_sorted[start + j] : 0.0f;
-                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64, n_pairs);
+                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64, n_pairs,
+                                         kern);
             } else {
                 stack = (lane == sp) ? 2 * node + 1 : stack;
                 ++sp;
@@ -790,7 +819,8 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
         for (int i = 0; i < n_list; ++i) {
             const uint32_t si = L.list[lane][i];
             const float4 c = spos[si];
-            if (norm2(qx, qy, qz, c.x, c.y, c.z) < tau) continue;
+            const float d2c = norm2(qx, qy, qz, c.x, c.y, c.z);
+            if (d2c < tau) continue;
             const uint32_t idx = __float_as_uint(c.w);
             int r = c_lt;
             for (int k = 0; k < n_list; ++k) {
@@ -798,7 +828,8 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
                 if (norm2(qx, qy, qz, o.x, o.y, o.z) == tau && __float_as_uint(o.w) < idx) ++r;
             }
             if (r < K - 1 && dens64)
-                unsafeAtomicAdd(&dens64[si], static_cast<double>(tent_weight(qx, qy, qz, c.x, c.y, c.z, hinv)));
+                unsafeAtomicAdd(&dens64[si],
+                                static_cast<double>(dens_weight(kern, qx, qy, qz, c.x, c.y, c.z, hinv, d2c)));
             if (r == K - 1) kk = make_key(tau, idx);
         }
     }
@@ -1006,7 +1037,7 @@ __global__ void kmax_level_kernel(float* __restrict__ kmax, int64_t first, int64
 // nbr_search.cpp:81 does.  One query per lane, packet walk as the search.
 __global__ __launch_bounds__(kKnnThreads) void dens_gather_kernel(
     const float4* __restrict__ spos, const uint64_t* __restrict__ kkey, const float* __restrict__ rk, int64_t n,
-    const Box* __restrict__ nodes, const float* __restrict__ kmax, int64_t nleaf_pad, float* __restrict__ dens)
+    const Box* __restrict__ nodes, const float* __restrict__ kmax, int64_t nleaf_pad, float* __restrict__ dens, int kern)
 {
     __shared__ float4 stage_p[kWavesPerBlock][kWave];
     __shared__ float4 stage_k[kWavesPerBlock][kWave];   // (kkey lo, kkey hi, hinv, -)
@@ -1047,7 +1078,7 @@ __global__ __launch_bounds__(kKnnThreads) void dens_gather_kernel(
                 const float d2 = norm2(p.x, p.y, p.z, qx, qy, qz);
                 const uint64_t kp = (static_cast<uint64_t>(__float_as_uint(pk.y)) << 32) | __float_as_uint(pk.x);
                 if (make_key(d2, qidx) < kp)
-                    acc += static_cast<double>(tent_weight(p.x, p.y, p.z, qx, qy, qz, pk.z));
+                    acc += static_cast<double>(dens_weight(kern, p.x, p.y, p.z, qx, qy, qz, pk.z, d2));
             }
             wave_lds_fence();
         } else {
@@ -1094,7 +1125,8 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     if (method == kKnnInsertion) {
         knn_dens_kernel<kSlots><<<ins_blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size,
                                                                   init_leaves, rk_sorted, kkey_sorted, inv_perm,
-                                                                  dens64, nbr_out_orig, stats, nullptr);
+                                                                  dens64, nbr_out_orig, stats, nullptr,
+                                                                  scratch.kernel);
         return hipGetLastError();
     }
     if (leaf_size < 1 || leaf_size > kWave) return hipErrorInvalidValue;
@@ -1133,7 +1165,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
 #define ASM_COLLECT(M, PAD, ALL)                                                                                \
     knn_collect_kernel<kSlots, M, PAD, ALL><<<col_blocks, kCollectThreads, 0, s>>>(                             \
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.wleaf, scratch.wleaf_n, scratch.tau,  \
-        rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats)
+        rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats, scratch.kernel)
 #define ASM_COLLECT_PAD(M, ALL)                  \
     if (leaf_size <= 8) ASM_COLLECT(M, 8, ALL);  \
     else if (leaf_size <= 16) ASM_COLLECT(M, 16, ALL); \
@@ -1153,7 +1185,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     mark_begin(2);
     knn_dens_kernel<kSlots><<<ins_blocks, kKnnThreads, 0, s>>>(spos, n, nodes, nleaf_pad, nleaf, leaf_size,
                                                               init_leaves, rk_sorted, kkey_sorted, inv_perm, dens64,
-                                                              nbr_out_orig, stats, scratch.wave_flags);
+                                                              nbr_out_orig, stats, scratch.wave_flags, scratch.kernel);
     ASM_HIP_CHECK(hipGetLastError());
     mark_end(2);
     return hipSuccess;
@@ -1174,12 +1206,12 @@ hipError_t launch_kmax(const uint64_t* kkey, int64_t n, int leaf_size, int64_t n
 }
 
 hipError_t launch_dens_gather(const float4* spos, const uint64_t* kkey, const float* rk, int64_t n, const Box* nodes,
-                              const float* kmax, int64_t nleaf_pad, float* dens, hipStream_t s)
+                              const float* kmax, int64_t nleaf_pad, float* dens, int kern, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int64_t waves = (n + kWave - 1) / kWave;
     const unsigned blocks = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    dens_gather_kernel<<<blocks, kKnnThreads, 0, s>>>(spos, kkey, rk, n, nodes, kmax, nleaf_pad, dens);
+    dens_gather_kernel<<<blocks, kKnnThreads, 0, s>>>(spos, kkey, rk, n, nodes, kmax, nleaf_pad, dens, kern);
     return hipGetLastError();
 }
 

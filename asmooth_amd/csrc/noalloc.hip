@@ -167,7 +167,7 @@ extern "C" int asm_smooth_noalloc(int device, const asm_particle* d_particles, i
     NAC(launch_gather_sorted(reinterpret_cast<const ParticleRec*>(d_particles), perm, n, spos, sattr, inv_perm, s));
     NAC(launch_build_tree(spos, n, kLeaf, nleaf, nleaf_pad, nodes, groups, s));
     // Search, h and density (asm_neighbor_search, bound + collect walks).
-    KnnScratch kscr{wleaf, wleaf_n, wave_flags, tau, dens_pull, nullptr, nullptr, nullptr, groups, 0};
+    KnnScratch kscr{wleaf, wleaf_n, wave_flags, tau, dens_pull, nullptr, nullptr, nullptr, groups, 0, kKernTent};
     NAC(launch_fill_zero_f64(dens64, n, s));
     NAC(launch_knn(spos, n, nodes, nleaf_pad, nleaf, kLeaf, rk, kkey, inv_perm, dens64, nullptr, nullptr, kscr,
                    kKnnBoundCollect, s));
@@ -189,6 +189,7 @@ extern "C" int asm_smooth_noalloc(int device, const asm_particle* d_particles, i
         a.ntx = ntx;
         a.periodic = 0;
         a.accum = kAccumFp64;
+        a.kern = kKernTent;
         a.list_cap = static_cast<uint32_t>(kMaxList);
         NAC(launch_fill_zero_u32(tile_cnt, ntiles, s));
         NAC(launch_deposit_count(a, tile_cnt, s));

@@ -45,6 +45,12 @@ def olib() -> C.CDLL:
         L.orc_density.argtypes = [vp, i64, vp, i32, vp, vp]
         L.orc_ker_integ.restype = f32
         L.orc_ker_integ.argtypes = [f32, f32]
+        L.orc_spline.restype = f32
+        L.orc_spline.argtypes = [vp, vp, f32]
+        L.orc_spline_integ.restype = f32
+        L.orc_spline_integ.argtypes = [f32, f32]
+        L.orc_set_kernel.restype = None
+        L.orc_set_kernel.argtypes = [i32]
         L.orc_assign_on_mesh.restype = i32
         L.orc_assign_on_mesh.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, vp]
         L.orc_assign_on_global_mesh.restype = i32
@@ -128,13 +134,62 @@ def knn(pos: np.ndarray, queries=None, k_slots: int = K_SLOTS, brute: bool = Fal
     return nbr, d2, rk
 
 
-def density(pos: np.ndarray, nbr: np.ndarray, rk: np.ndarray) -> np.ndarray:
+class kernel:
+    """Context manager selecting the oracle's smoothing kernel for density()
+    and assign_on_mesh(): 0 the tent (default), 1 the cubic spline."""
+
+    def __init__(self, k: int):
+        self.k = int(k)
+
+    def __enter__(self):
+        olib().orc_set_kernel(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        olib().orc_set_kernel(0)
+
+
+def density(pos: np.ndarray, nbr: np.ndarray, rk: np.ndarray, kern: int = 0) -> np.ndarray:
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
     nbr = np.ascontiguousarray(nbr, np.int32)
     rk = np.ascontiguousarray(rk, np.float32)
     d = np.zeros(len(pos), np.float64)
-    olib().orc_density(_p(pos), len(pos), _p(nbr), nbr.shape[1], _p(rk), _p(d))
+    with kernel(kern):
+        olib().orc_density(_p(pos), len(pos), _p(nbr), nbr.shape[1], _p(rk), _p(d))
     return d
+
+
+def spline(x, y, hinv) -> float:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return olib().orc_spline(_p(x), _p(y), float(hinv))
+
+
+def spline_integ(x1: float, x2: float) -> float:
+    return olib().orc_spline_integ(float(x1), float(x2))
+
+
+_rs = None
+
+
+def ref_spline_ker(x, y, hinv) -> float:
+    """The reference's own spline_ker (nbr_search.cpp:39-53), compiled from its
+    source into _ref/libasmref_spline.so (where the reference tree exists)."""
+    global _rs
+    if _rs is None:
+        path = os.path.join(HERE, "_ref", "libasmref_spline.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing (built only where the reference tree exists)")
+        _rs = C.CDLL(path)
+        _rs.ref_spline_ker.restype = C.c_float
+        _rs.ref_spline_ker.argtypes = [C.c_void_p, C.c_void_p, C.c_float]
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return _rs.ref_spline_ker(_p(x), _p(y), float(hinv))
+
+
+def have_ref_spline() -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", "libasmref_spline.so"))
 
 
 def ker_integ(x1: float, x2: float) -> float:
@@ -153,20 +208,21 @@ def norm2(x, y) -> float:
     return olib().orc_norm2(_p(x), _p(y))
 
 
-def assign_on_mesh(pos, vel, dens, rk, igrp, L: float, nc: int, sign: int = -1, periodic_coord=None):
+def assign_on_mesh(pos, vel, dens, rk, igrp, L: float, nc: int, sign: int = -1, periodic_coord=None, kern: int = 0):
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
     vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
     dens = np.ascontiguousarray(dens, np.float32)
     rk = np.ascontiguousarray(rk, np.float32)
     igrp = np.ascontiguousarray(igrp, np.int32)
     mesh = np.zeros(6 * nc ** 3, np.float64)
-    if periodic_coord is None:
-        rc = olib().orc_assign_on_mesh(_p(pos), _p(vel), _p(dens), _p(rk), _p(igrp), len(pos), float(L),
-                                       int(nc), int(sign), _p(mesh))
-    else:
-        c3 = np.ascontiguousarray(periodic_coord, np.int32)
-        rc = olib().orc_assign_on_global_mesh(_p(pos), _p(vel), _p(dens), _p(rk), _p(igrp), len(pos),
-                                              float(L), int(nc), int(sign), _p(c3), _p(mesh))
+    with kernel(kern):
+        if periodic_coord is None:
+            rc = olib().orc_assign_on_mesh(_p(pos), _p(vel), _p(dens), _p(rk), _p(igrp), len(pos), float(L),
+                                           int(nc), int(sign), _p(mesh))
+        else:
+            c3 = np.ascontiguousarray(periodic_coord, np.int32)
+            rc = olib().orc_assign_on_global_mesh(_p(pos), _p(vel), _p(dens), _p(rk), _p(igrp), len(pos),
+                                                  float(L), int(nc), int(sign), _p(c3), _p(mesh))
     if rc != 0:
         raise RuntimeError(f"orc_assign_on_mesh failed ({rc})")
     return mesh

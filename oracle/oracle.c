@@ -10,6 +10,9 @@
 #include "oracle.h"
 
 #include <math.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846   /* the double <math.h> gives under POSIX */
+#endif
 #include <float.h>
 #include <stdlib.h>
 #include <string.h>
@@ -170,6 +173,59 @@ float orc_tent(const float* x, const float* y, float hinv)
     return w;
 }
 
+/* nbr_search.cpp:39-53 (spline_ker, defined there but not called): the cubic
+ * spline 8/(pi h^3) (1 - 6 r^2 + 6 r^3) for r < 1/2, 8/(pi h^3) 2 (1 - r)^3
+ * beyond, r = |x - y| / h.  The factor is evaluated in FP64 (M_PI is a double)
+ * and rounded once; r = sqrtf(d2) * hinv; the pieces in FP32, left to right. */
+float orc_spline(const float* x, const float* y, float hinv)
+{
+    const double hd = (double)hinv;
+    const float fac = (float)(((8.0 / M_PI) * hd) * hd * hd);
+    const float r = sqrtf(orc_norm2(x, y)) * hinv;
+    if (r < 0.5f) {
+        const float r6 = 6.0f * r;
+        return fac * ((1.0f - r6 * r) + (r6 * r) * r);
+    }
+    const float r1 = 1.0f - r;
+    return fac * (((2.0f * r1) * r1) * r1);
+}
+
+/* Density and mesh kernel of the oracle: 0 the tent (the reference's), 1 the
+ * cubic spline (orc_spline for the density; the separable cubic B-spline,
+ * orc_spline_integ, on the mesh). */
+static int g_kernel = 0;
+
+void orc_set_kernel(int kernel)
+{
+    g_kernel = kernel;
+}
+
+/* The separable cubic B-spline of half-width 1 normalised to one,
+ * w(t) = 4/3 (1 - 6 t^2 + 6 |t|^3) for |t| < 1/2 and 8/3 (1 - |t|)^3 up to 1:
+ * its antiderivative on [0, inf), G(u) = 4/3 (u - 2 u^3 + 3/2 u^4) below 1/2,
+ * 1/2 - 2/3 (1 - u)^4 up to 1, then 1/2. */
+static float spline_g(float u)
+{
+    if (u >= 1.0f) return 0.5f;
+    if (u < 0.5f) {
+        const float u2 = u * u;
+        const float a = 2.0f * (u2 * u);
+        const float b = 1.5f * (u2 * u2);
+        return (4.0f / 3.0f) * ((u - a) + b);
+    }
+    const float r1 = 1.0f - u;
+    const float r2 = r1 * r1;
+    return 0.5f - (2.0f / 3.0f) * (r2 * r2);
+}
+
+/* Its integral over [x1, x2], as orc_ker_integ is the tent's. */
+float orc_spline_integ(float x1, float x2)
+{
+    const float fac1 = (float)(-1 + 2 * (x1 > 0.0f));
+    const float fac2 = (float)(-1 + 2 * (x2 > 0.0f));
+    return fac2 * spline_g(fac2 * x2) - fac1 * spline_g(fac1 * x1);
+}
+
 /* coarse_mesh.cpp:10-15: antiderivative of the unit tent on [0, inf). */
 static float tent_f(float x)
 {
@@ -182,6 +238,12 @@ float orc_ker_integ(float x1, float x2)
     const float fac1 = (float)(-1 + 2 * (x1 > 0.0f));
     const float fac2 = (float)(-1 + 2 * (x2 > 0.0f));
     return fac2 * tent_f(fac2 * x2) - fac1 * tent_f(fac1 * x1);
+}
+
+/* The cell integral of the selected kernel. */
+static float cell_integ(float x1, float x2)
+{
+    return g_kernel == 1 ? orc_spline_integ(x1, x2) : orc_ker_integ(x1, x2);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -395,7 +457,9 @@ void orc_density(const float* pos, int64_t n, const int32_t* nbr, int k_slots,
          * (nbr_search.cpp:78: slots 1..k-1 of the descending list). */
         for (int s = 0; s < k_slots - 1; ++s) {
             const int32_t q = nbr[p * k_slots + s];
-            dens64[q] += (double)orc_tent(pos + 3 * p, pos + 3 * (int64_t)q, hinv);
+            const float* x = pos + 3 * p;
+            const float* y = pos + 3 * (int64_t)q;
+            dens64[q] += (double)(g_kernel == 1 ? orc_spline(x, y, hinv) : orc_tent(x, y, hinv));
         }
     }
 }
@@ -446,13 +510,13 @@ int orc_assign_on_mesh(const float* pos, const float* vel, const float* dens,
         const int igm = (sign * igrp[j] <= 0);
         for (int iz = left[2]; iz < right[2]; ++iz) {
             float s1 = ((float)iz * dx - x[2]) * hinv;
-            const float kz = orc_ker_integ(s1, s1 + ds);
+            const float kz = cell_integ(s1, s1 + ds);
             for (int iy = left[1]; iy < right[1]; ++iy) {
                 s1 = ((float)iy * dx - x[1]) * hinv;
-                const float ky = orc_ker_integ(s1, s1 + ds);
+                const float ky = cell_integ(s1, s1 + ds);
                 for (int ix = left[0]; ix < right[0]; ++ix) {
                     s1 = ((float)ix * dx - x[0]) * hinv;
-                    const float kx = orc_ker_integ(s1, s1 + ds);
+                    const float kx = cell_integ(s1, s1 + ds);
                     const int64_t index = iz * nc2 + (int64_t)iy * nc + ix;
                     float ker3 = kx * ky;
                     ker3 = ker3 * kz;
@@ -495,13 +559,13 @@ int orc_assign_on_global_mesh(const float* pos, const float* vel, const float* d
         const int igm = (sign * igrp[j] <= 0);
         for (int iz = left[2]; iz < right[2]; ++iz) {
             float s1 = ((float)iz * dx - x[2]) * hinv;
-            const float kz = orc_ker_integ(s1, s1 + ds);
+            const float kz = cell_integ(s1, s1 + ds);
             for (int iy = left[1]; iy < right[1]; ++iy) {
                 s1 = ((float)iy * dx - x[1]) * hinv;
-                const float ky = orc_ker_integ(s1, s1 + ds);
+                const float ky = cell_integ(s1, s1 + ds);
                 for (int ix = left[0]; ix < right[0]; ++ix) {
                     s1 = ((float)ix * dx - x[0]) * hinv;
-                    const float kx = orc_ker_integ(s1, s1 + ds);
+                    const float kx = cell_integ(s1, s1 + ds);
                     const int64_t index = (int64_t)((iz + nc) % nc) * nc2 +
                                           (int64_t)((iy + nc) % nc) * nc + ((ix + nc) % nc);
                     float ker3 = kx * ky;

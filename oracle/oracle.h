@@ -11,8 +11,9 @@
  * Reference citations are relative to the asmooth source tree.
  *
  * Pinning (what checks each function against something other than itself):
- *   orc_buffered_set    - compared with the reference binary's np_with_buffers
- *                         count and images; closed form (1+2bf)^3 ratio.
+ *   orc_buffered_set    - the compiled reference's exchange_buffer on one rank
+ *                         (tests/golden/reference_mpi.npz), order included;
+ *                         closed form (1+2bf)^3 ratio.
  *   orc_knn             - pinned by orc_knn_brute (O(N^2)), by scipy cKDTree
  *                         (library kNN, FP64 metric) and by the compiled
  *                         reference (oracle/_ref) neighbour lists and h.
@@ -21,10 +22,18 @@
  *   orc_assign_on_mesh  - partition of unity, single-particle product of 1-D
  *                         integrals, numerical quadrature of the tent, mass
  *                         conservation, the compiled reference mesh.
- *   orc_assign_on_global_mesh - periodic partition of unity, equality with
+ *   orc_assign_on_global_mesh - the compiled reference's assign_on_global_mesh
+ *                         at every node coordinate of a 2^3 decomposition
+ *                         (tests/golden/reference_mpi.npz); a wrapped
+ *                         single-particle outer product; equality with
  *                         orc_assign_on_mesh for particles whose support lies
- *                         inside the cube (no reference caller exists).
+ *                         inside the cube.
  *   orc_mark_halo       - compiled reference mark_halo_particles.
+ *   orc_spline          - the compiled reference's spline_ker (nbr_search.cpp
+ *                         compiled in place), bit for bit; W(0) = 8/(pi h^3),
+ *                         continuity at r = 1/2, unit integral by quadrature.
+ *   orc_spline_integ    - the integral of its 1-D B-spline: scipy quadrature,
+ *                         partition of unity, symmetry.
  *   orc_normalize_*     - algebraic identities; compiled reference writer.
  */
 #ifndef ASMOOTH_ORACLE_H
@@ -62,6 +71,12 @@ void orc_density(const float* pos, int64_t n, const int32_t* nbr, int k_slots,
 
 /* Cell integral of the unit tent (coarse_mesh.cpp:10-22). */
 float orc_ker_integ(float x1, float x2);
+/* The cubic spline option: the density kernel of nbr_search.cpp:39-53, the
+ * cell integral of the separable cubic B-spline, and the selection (0 tent,
+ * 1 spline) used by orc_density and the two depositions. */
+float orc_spline(const float* x, const float* y, float hinv);
+float orc_spline_integ(float x1, float x2);
+void orc_set_kernel(int kernel);
 
 /* Mesh deposition (coarse_mesh.cpp:45-127), FP32 terms, FP64 sums.
  * mesh64: 6*nc^3 = n | halo | c | v(xyz interleaved). Accumulates (caller zeroes). */

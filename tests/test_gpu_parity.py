@@ -401,3 +401,57 @@ def test_deterministic_density_mode(ctx):
     assert np.array_equal(runs[0]["meshes"][16].view(np.int32), runs[1]["meshes"][16].view(np.int32))
     assert np.array_equal(runs[0]["rk"], ork)
     assert rel_err(runs[0]["dens"], od) <= DENS_RTOL
+
+
+@pytest.mark.parametrize("method,deterministic", [(0, 0), (1, 0), (0, 1)])
+def test_spline_kernel_matches_oracle(ctx, method, deterministic):
+    """ASM_OPT_KERNEL 1: the density scatter with the reference's cubic spline
+    (spline_ker, nbr_search.cpp:39-53; the oracle's is pinned to it bit for
+    bit) and the node and periodic meshes with the separable cubic B-spline,
+    against the oracle; h does not depend on the kernel.  Every search path:
+    the bound and collect walks, the insertion kernel and the ordered gather."""
+    xv, L, bf = small_set("blobs", 16, 21)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    pos, vel = np.ascontiguousarray(buf[:, :3]), np.ascontiguousarray(buf[:, 3:])
+    nbr, _, rk = O.knn(pos)
+    dens64 = O.density(pos, nbr, rk, kern=1)
+    vmax = float(np.max(np.abs(vel)))
+    ctx.set_option(A.OPT_KERNEL, 1)
+    ctx.set_option(A.OPT_KNN_METHOD, method)
+    ctx.set_option(A.OPT_DETERMINISTIC, deterministic)
+    try:
+        g = gpu_path(ctx, parts, nl, L, bf, [16, 7], keep_neighbors=False)
+        assert np.array_equal(g["rk"].view(np.int32), rk.view(np.int32))
+        assert rel_err(g["dens"], dens64, floor=1e-30) <= DENS_RTOL
+        tent = O.density(pos, nbr, rk)
+        assert rel_err(g["dens"], tent, floor=1e-30) > 1e-3   # a different kernel than the tent
+        d32 = g["dens"]
+        for nc in (16, 7):
+            om = O.assign_on_mesh(pos, vel, d32, rk, np.zeros(len(pos), np.int32), L, nc, -1, kern=1)
+            _check_mesh(g["meshes"][nc], om, nc, vmax)
+        ctx.assign_on_global_mesh(12, -1, (1, 0, 1))
+        om = O.assign_on_mesh(pos, vel, d32, rk, np.zeros(len(pos), np.int32), L, 12, -1, periodic_coord=(1, 0, 1),
+                              kern=1)
+        _check_mesh(ctx.get_mesh(12), om, 12, vmax)
+    finally:
+        ctx.set_option(A.OPT_KERNEL, 0)
+        ctx.set_option(A.OPT_KNN_METHOD, 0)
+        ctx.set_option(A.OPT_DETERMINISTIC, 0)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_spline_kernel_with_ties_matches_oracle(ctx, method):
+    """The spline density on the tie-heavy quantized grid: the terms at the
+    33rd distance, ranked by index, go through the spline as well."""
+    pos = _tie_heavy_sets()["quantized30"]
+    onbr, _, ork = O.knn(pos)
+    odens = O.density(pos, onbr, ork, kern=1)
+    ctx.set_option(A.OPT_KERNEL, 1)
+    try:
+        for keep, det in ((0, 0), (1, 0), (0, 1)):
+            rk, dens, _ = _knn_run(ctx, pos, method, deterministic=det, keep=keep)
+            assert np.array_equal(rk.view(np.int32), ork.view(np.int32))
+            assert rel_err(dens, odens) <= DENS_RTOL, (keep, det)
+    finally:
+        ctx.set_option(A.OPT_KERNEL, 0)
