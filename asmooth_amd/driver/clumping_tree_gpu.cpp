@@ -255,9 +255,13 @@ int main(int argc, char** argv)
             std::printf("n_so_halo %lld %lld\n", static_cast<long long>(nh), static_cast<long long>(nhb));
             std::printf("so_halo_local %lld %g\n", static_cast<long long>(np_halo), m_halo);
             std::printf("check halo_mass %s %lld %g\n", z.c_str(), static_cast<long long>(np_halo), m_halo);
-            if (m_halo > 100.0 && !(0.80 < np_halo / m_halo && np_halo / m_halo < 1.20))
-                std::fprintf(stderr, "Warning: halo particle count / mass ratio %g outside (0.8, 1.2)\n",
+            // clumping_tree_main.cpp:507-509 asserts this (the run stops)
+            if (m_halo > 100.0 && !(0.80 < np_halo / m_halo && np_halo / m_halo < 1.20)) {
+                std::fprintf(stderr, "Error: halo particle count / mass ratio %g outside (0.8, 1.2)\n",
                              np_halo / m_halo);
+                asm_destroy(ctx);
+                return 1;
+            }
         } else {
             std::printf("haloes not excised\n");
         }
@@ -268,13 +272,19 @@ int main(int argc, char** argv)
             if ((rc = asm_assign_on_mesh(ctx, nc, -1))) return die(ctx, "asm_assign_on_mesh", rc);
             double tot[2];
             if ((rc = asm_mesh_total(ctx, nc, tot))) return die(ctx, "asm_mesh_total", rc);
-            // check_total_mass (clumping_tree_main.cpp:578-600)
+            // check_total_mass (clumping_tree_main.cpp:578-600): against
+            // np_total, the node's particle count before the buffer was
+            // trimmed (exchange_buffer.cpp:156-163); the reference asserts
+            // (:597-598), so the run stops before this mesh is written.
             const double m_total = tot[0] + tot[1];
-            std::printf("check total_mass %4d %15.1f %15.1f %lld\n", nc, tot[0], m_total,
-                        static_cast<long long>(np_local));
-            if (!(0.999 < m_total / static_cast<double>(np_local) && m_total / static_cast<double>(np_local) < 1.001))
-                std::fprintf(stderr, "Warning: total mass %g deviates from np %lld by more than 1e-3\n", m_total,
-                             static_cast<long long>(np_local));
+            std::printf("check total_mass %4d %15.1f %15.1f %lld\n", nc, tot[0], m_total, static_cast<long long>(np));
+            const double ratio = m_total / static_cast<double>(np);
+            if (!(0.999 < ratio && ratio < 1.001)) {
+                std::fprintf(stderr, "Error: total mass %.1f deviates from np_total %lld by more than 1e-3 (nc %d)\n",
+                             m_total, static_cast<long long>(np), nc);
+                asm_destroy(ctx);
+                return 1;
+            }
             if ((rc = asm_normalize_mesh(ctx, nc))) return die(ctx, "asm_normalize_mesh", rc);
             asm_synchronize(ctx);
             double b = now_s();

@@ -228,9 +228,9 @@ def _write_zip(d, z, inode, nd, counts, b, s, v_r2i):
     counts[counts >= 255].astype(np.int32).tofile(f"{base}3_{inode}.dat")
 
 
-def _zip_case(seed, nd, big=()):
+def _zip_case(seed, nd, big=(), lam=3.0):
     rng = np.random.default_rng(seed)
-    counts = rng.poisson(3.0, nd ** 3).astype(np.int64)
+    counts = rng.poisson(lam, nd ** 3).astype(np.int64)
     for c, k in big:
         counts[c] = k
     n = int(counts.sum())
