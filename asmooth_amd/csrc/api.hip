@@ -1,6 +1,7 @@
 // C ABI of the device path: context, device buffers, call sequencing, timers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -608,6 +609,96 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     return ASM_OK;
 }
 
+// Every node mesh of an nc list together (FP64 sums, the default): one count
+// and one fill pass over the particles for all meshes, one scan of the
+// concatenated tile counters, one read-back of the list total (none while a
+// batch records) and one launch over all meshes' tiles, coarse meshes first.
+// Other accumulation modes take the per-mesh path.
+int deposit_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
+{
+    std::vector<int> list_nc;
+    for (int i = 0; i < n_nc; ++i) {
+        if (ncs[i] <= 0 || ncs[i] > 2048) return fail(ctx, ASM_ERR_ARG, "nc %d outside 1..2048", ncs[i]);
+        if (std::find(list_nc.begin(), list_nc.end(), ncs[i]) == list_nc.end()) list_nc.push_back(ncs[i]);
+    }
+    if (ctx->deposit_accum != kAccumFp64 || static_cast<int>(list_nc.size()) > kMaxMeshes) {
+        for (int nc : list_nc) {
+            const int rc = deposit_one(ctx, nc, sign, 0, nullptr);
+            if (rc) return rc;
+        }
+        return ASM_OK;
+    }
+    if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "mesh assignment before asm_neighbor_search");
+    HIPC(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    std::sort(list_nc.begin(), list_nc.end());   // coarse meshes (long tiles) first
+    MultiDeposit J{};
+    J.n = static_cast<int>(list_nc.size());
+    uint64_t tiles = 0;
+    for (int j = 0; j < J.n; ++j) {
+        const int nc = list_nc[j];
+        const int64_t nmesh = static_cast<int64_t>(nc) * nc * nc;
+        MeshDev& m = ctx->meshes[nc];
+        HIPC(m.mesh.ensure(static_cast<size_t>(6 * nmesh)));
+        DepositArgs& a = J.a[j];
+        a.spos = ctx->spos.p;
+        a.sattr = ctx->sattr.p;
+        a.rk = ctx->rk.p;
+        a.dens = ctx->dens.p;
+        a.n = ctx->n;
+        a.boxsize = ctx->boxsize;
+        a.nc = nc;
+        a.sign = sign;
+        const auto it = ctx->tile_for_nc.find(nc);
+        a.tile = it != ctx->tile_for_nc.end() ? it->second : ctx->tile;
+        a.ntx = (nc + a.tile - 1) / a.tile;
+        a.periodic = 0;
+        a.accum = kAccumFp64;
+        a.kern = ctx->kernel;
+        J.mesh[j] = m.mesh.p;
+        J.tile_base[j] = static_cast<uint32_t>(tiles);
+        tiles += static_cast<uint64_t>(a.ntx) * a.ntx * a.ntx;
+        if (tiles > 0x7fffffffull) return fail(ctx, ASM_ERR_CAPACITY, "the nc list gives too many tiles");
+    }
+    J.tile_base[J.n] = static_cast<uint32_t>(tiles);
+    const int64_t ntiles = static_cast<int64_t>(tiles);
+    HIPC(ctx->tile_cnt.ensure(ntiles));
+    HIPC(ctx->tile_start.ensure(ntiles));
+    HIPC(ctx->tile_cur.ensure(ntiles));
+    HIPC(ctx->scan_total.ensure(1));
+    HIPC(ctx->scan_tmp.ensure(static_cast<size_t>(scan_scratch_entries(ntiles))));
+    {
+        StageTimer t(ctx, T_DEPOSIT_LIST);
+        HIPC(launch_fill_zero_u32(ctx->tile_cnt.p, ntiles, s));
+        HIPC(launch_multi_count(J, ctx->tile_cnt.p, s));
+        HIPC(hipMemcpyAsync(ctx->tile_start.p, ctx->tile_cnt.p, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToDevice, s));
+        HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, ctx->scan_tmp.p, s));
+    }
+    if (!ctx->capturing) {
+        uint64_t total = 0;
+        HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        if (total > 0xffffffffull)
+            return fail(ctx, ASM_ERR_CAPACITY,
+                        "the nc list needs %llu tile-list entries, more than 2^32 - 1; use larger tiles",
+                        static_cast<unsigned long long>(total));
+        HIPC(ctx->list.ensure(total > 0 ? total : 1));
+    }
+    const uint32_t cap = static_cast<uint32_t>(ctx->list.cap < 0xffffffffull ? ctx->list.cap : 0xffffffffull);
+    for (int j = 0; j < J.n; ++j) J.a[j].list_cap = cap;
+    if (ctx->overflow_flag) HIPC(launch_flag_overflow(ctx->scan_total.p, cap, ctx->overflow_flag, s));
+    {
+        StageTimer t(ctx, T_DEPOSIT_LIST);
+        HIPC(launch_fill_zero_u32(ctx->tile_cur.p, ntiles, s));
+        HIPC(launch_multi_fill(J, ctx->tile_start.p, ctx->tile_cur.p, ctx->list.p, s));
+    }
+    {
+        StageTimer t(ctx, T_DEPOSIT_TILES);
+        HIPC(launch_multi_tiles(J, ctx->tile_start.p, ctx->tile_cnt.p, ctx->list.p, s));
+    }
+    return ASM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -621,11 +712,7 @@ int asm_assign_on_mesh(asm_context* ctx, int nc, int sign)
 int asm_assign_on_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
 {
     if (!ctx || !ncs || n_nc <= 0) return fail(ctx, ASM_ERR_ARG, "empty nc list");
-    for (int i = 0; i < n_nc; ++i) {
-        int rc = deposit_one(ctx, ncs[i], sign, 0, nullptr);
-        if (rc) return rc;
-    }
-    return ASM_OK;
+    return deposit_meshes(ctx, ncs, n_nc, sign);
 }
 
 int asm_assign_on_global_mesh(asm_context* ctx, int nc, int sign, const int coord[3])
@@ -1068,10 +1155,9 @@ int run_cube(asm_batch::Cube& c, double* tot)
     if (ctx->overflow_flag) HIPC(launch_fill_zero_u32(ctx->overflow_flag, 1, ctx->stream));
     int rc = asm_tree_construct(ctx, c.left, c.right, nullptr);
     if (!rc) rc = asm_neighbor_search(ctx, kSlots - 1);
+    if (!rc) rc = deposit_meshes(ctx, c.ncs.data(), static_cast<int>(c.ncs.size()), c.sign);
     for (size_t i = 0; i < c.ncs.size() && !rc; ++i) {
         const int nc = c.ncs[i];
-        rc = deposit_one(ctx, nc, c.sign, 0, nullptr);
-        if (rc) break;
         MeshDev& m = ctx->meshes[nc];
         HIPC(ctx->red_partial.ensure(4 * 1024));
         HIPC(launch_mesh_reduce(m.mesh.p, nc, ctx->red_partial.p, tot + 4 * i, ctx->stream));

@@ -286,14 +286,14 @@ __global__ void deposit_count_kernel(DepositArgs a, uint32_t* __restrict__ tile_
 constexpr int kListThreads = 256;
 constexpr int kListTable = 1024;
 
+// One mesh's pass of a workgroup; cnt, base and bb are the workgroup's LDS.
+// Every barrier inside is reached by all threads or by none (the early
+// returns are uniform, or taken where no barrier follows).
 template <bool kFill>
-__global__ __launch_bounds__(kListThreads) void node_list_kernel(DepositArgs a, uint32_t* __restrict__ counters,
-                                                                 const uint32_t* __restrict__ tile_start,
-                                                                 uint32_t* __restrict__ list)
+__device__ __forceinline__ void node_list_body(const DepositArgs& a, uint32_t* __restrict__ counters,
+                                               const uint32_t* __restrict__ tile_start, uint32_t* __restrict__ list,
+                                               uint32_t* cnt, uint32_t* base, int* bb)
 {
-    __shared__ uint32_t cnt[kListTable];
-    __shared__ uint32_t base[kFill ? kListTable : 1];
-    __shared__ int bb[6];
     const int tid = threadIdx.x;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kListThreads + tid;
     int t0[3] = {0, 0, 0}, t1[3] = {-1, -1, -1};
@@ -383,6 +383,36 @@ __global__ __launch_bounds__(kListThreads) void node_list_kernel(DepositArgs a, 
                     const uint32_t pos = base[k] + atomicAdd(&cnt[k], 1u);
                     if (pos < a.list_cap) list[pos] = static_cast<uint32_t>(i);
                 }
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kListThreads) void node_list_kernel(DepositArgs a, uint32_t* __restrict__ counters,
+                                                                 const uint32_t* __restrict__ tile_start,
+                                                                 uint32_t* __restrict__ list)
+{
+    __shared__ uint32_t cnt[kListTable];
+    __shared__ uint32_t base[kFill ? kListTable : 1];
+    __shared__ int bb[6];
+    node_list_body<kFill>(a, counters, tile_start, list, cnt, base, bb);
+}
+
+// Every mesh of an nc list in one pass over the particles: the workgroup's
+// particles (already in registers and cache after the first mesh) are listed
+// in each mesh's tiles in turn.  Tile t of mesh j is t + J.tile_base[j] in
+// the concatenated counters, starts and list.
+template <bool kFill>
+__global__ __launch_bounds__(kListThreads) void multi_list_kernel(MultiDeposit J, uint32_t* __restrict__ counters,
+                                                                  const uint32_t* __restrict__ tile_start,
+                                                                  uint32_t* __restrict__ list)
+{
+    __shared__ uint32_t cnt[kListTable];
+    __shared__ uint32_t base[kFill ? kListTable : 1];
+    __shared__ int bb[6];
+    for (int j = 0; j < J.n; ++j) {
+        __syncthreads();   // the previous mesh's readers of bb, cnt and base are done
+        const uint32_t off = J.tile_base[j];
+        node_list_body<kFill>(J.a[j], counters + off, kFill ? tile_start + off : nullptr, list, cnt, base, bb);
+    }
 }
 
 __global__ void deposit_fill_kernel(DepositArgs a, const uint32_t* __restrict__ tile_start,
@@ -554,9 +584,11 @@ constexpr int kWStride = 3 * kTileMax + 1;   // odd row stride: entries fall in 
 constexpr int kCellsPerLane = 4;             // cells of one box a lane takes per sweep
 
 template <typename Acc>
-__global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
-    DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
-    const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
+__device__ __forceinline__ void tile_packed_body(const DepositArgs& a, int64_t tile,
+                                                 const uint32_t* __restrict__ tile_start,
+                                                 const uint32_t* __restrict__ tile_cnt,
+                                                 const uint32_t* __restrict__ list, float* __restrict__ mesh,
+                                                 int accumulate)
 {
     __shared__ Acc acc[6][kPadCells];
     __shared__ float wts[kDepWaves][kWave * kWStride];
@@ -567,7 +599,6 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
     float* const W = wts[wid];
     int* const map = smap[wid];
     const int T = a.tile;
-    const int64_t tile = blockIdx.x;
     const int tx = static_cast<int>(tile % a.ntx);
     const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
     const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
@@ -681,6 +712,29 @@ __global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
     }
     __syncthreads();
     flush_tile(a, acc, cx0, cy0, cz0, mesh, accumulate);
+}
+
+template <typename Acc>
+__global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
+    DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
+    const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
+{
+    tile_packed_body<Acc>(a, blockIdx.x, tile_start, tile_cnt, list, mesh, accumulate);
+}
+
+// The tiles of every mesh of an nc list in one launch (FP64 sums): workgroup
+// w runs tile w - J.tile_base[j] of the mesh j whose range holds w.  The
+// meshes come in the caller's order; with coarse meshes first, their few
+// long tiles start before the many short ones of the fine meshes.
+__global__ __launch_bounds__(kPackThreads) void multi_tile_kernel(MultiDeposit J, const uint32_t* __restrict__ tile_start,
+                                                                  const uint32_t* __restrict__ tile_cnt,
+                                                                  const uint32_t* __restrict__ list)
+{
+    const uint32_t w = blockIdx.x;
+    int j = 0;
+    while (j + 1 < J.n && w >= J.tile_base[j + 1]) ++j;
+    const uint32_t off = J.tile_base[j];
+    tile_packed_body<double>(J.a[j], w - off, tile_start + off, tile_cnt + off, list, J.mesh[j], 0);
 }
 
 // Precision study: the tile's sums in FP32 with Kahan compensation.  One
@@ -927,6 +981,37 @@ hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start,
     }
     const int th = 256;
     deposit_fill_kernel<<<static_cast<unsigned>((a.n + th - 1) / th), th, 0, s>>>(a, tile_start, tile_cur, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_count(const MultiDeposit& J, uint32_t* tile_cnt, hipStream_t s)
+{
+    if (J.n < 1 || J.n > kMaxMeshes || J.a[0].n <= 0) return J.n < 1 || J.n > kMaxMeshes ? hipErrorInvalidValue
+                                                                                          : hipSuccess;
+    multi_list_kernel<false><<<static_cast<unsigned>((J.a[0].n + kListThreads - 1) / kListThreads), kListThreads, 0,
+                               s>>>(J, tile_cnt, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_fill(const MultiDeposit& J, const uint32_t* tile_start, uint32_t* tile_cur, uint32_t* list,
+                             hipStream_t s)
+{
+    if (J.n < 1 || J.n > kMaxMeshes || J.a[0].n <= 0) return J.n < 1 || J.n > kMaxMeshes ? hipErrorInvalidValue
+                                                                                          : hipSuccess;
+    multi_list_kernel<true><<<static_cast<unsigned>((J.a[0].n + kListThreads - 1) / kListThreads), kListThreads, 0,
+                              s>>>(J, tile_cur, tile_start, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_tiles(const MultiDeposit& J, const uint32_t* tile_start, const uint32_t* tile_cnt,
+                              const uint32_t* list, hipStream_t s)
+{
+    if (J.n < 1 || J.n > kMaxMeshes) return hipErrorInvalidValue;
+    for (int j = 0; j < J.n; ++j)
+        if (J.a[j].tile < 1 || J.a[j].tile > kTileMax || J.a[j].periodic) return hipErrorInvalidValue;
+    const uint32_t ntiles = J.tile_base[J.n];
+    if (ntiles == 0) return hipSuccess;
+    multi_tile_kernel<<<ntiles, kPackThreads, 0, s>>>(J, tile_start, tile_cnt, list);
     return hipGetLastError();
 }
 

@@ -159,6 +159,22 @@ struct DepositArgs {
 constexpr int kAccumFp64 = 0;      // FP64 LDS atomics
 constexpr int kAccumKahan32 = 1;   // FP32 with Kahan compensation, one wavefront per tile
 constexpr int kAccumAtomic32 = 2;  // plain FP32 LDS atomics
+// Every node mesh of an nc list at once (FP64 sums): mesh j's tiles are
+// tile_base[j] .. tile_base[j+1]-1 of one concatenated tile space, whose
+// counters, starts and list are shared; the particle fields of every a[j]
+// are the same.
+constexpr int kMaxMeshes = 8;
+struct MultiDeposit {
+    DepositArgs a[kMaxMeshes];
+    float* mesh[kMaxMeshes];
+    uint32_t tile_base[kMaxMeshes + 1];
+    int n;
+};
+hipError_t launch_multi_count(const MultiDeposit& J, uint32_t* tile_cnt, hipStream_t s);
+hipError_t launch_multi_fill(const MultiDeposit& J, const uint32_t* tile_start, uint32_t* tile_cur, uint32_t* list,
+                             hipStream_t s);
+hipError_t launch_multi_tiles(const MultiDeposit& J, const uint32_t* tile_start, const uint32_t* tile_cnt,
+                              const uint32_t* list, hipStream_t s);
 hipError_t launch_deposit_count(const DepositArgs& a, uint32_t* tile_cnt, hipStream_t s);
 hipError_t launch_deposit_fill(const DepositArgs& a, const uint32_t* tile_start, uint32_t* tile_cur,
                                uint32_t* list, hipStream_t s);

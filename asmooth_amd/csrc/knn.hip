@@ -620,10 +620,10 @@ struct CollectLds {
 //     c in its own walk when c sees o (d2 <= tau_c), so it is scattered with
 //     a global atomic only when d2 > tau_c;
 //   * terms at exactly tau_o depend on index order and are left to the end.
-template <int M, int P, bool kAll>
+template <int M, int P, bool kAll, int KERN>
 __device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int count, uint64_t mask, float4 c,
                                              float tc, int lane, float qx, float qy, float qz, float tau, float hinv,
-                                             double* __restrict__ dens64, unsigned long long& n_pairs, int kern)
+                                             double* __restrict__ dens64, unsigned long long& n_pairs)
 {
     constexpr int G = kWave / P;
     const int j = lane % P;
@@ -649,14 +649,14 @@ __device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int co
             if (d2 < o.w) {
                 if (dens64 && d2 > tc)
                     unsafeAtomicAdd(&dens64[start + j], static_cast<double>(dens_weight(
-                                                            kern, o.x, o.y, o.z, c.x, c.y, c.z, L.htab[k], d2)));
+                                                            KERN, o.x, o.y, o.z, c.x, c.y, c.z, L.htab[k], d2)));
             } else {
                 const uint32_t e = atomicAdd(&L.ecnt[own], 1u);
                 L.tidx[own] = __float_as_uint(c.w);
                 if (!kAll && e < static_cast<uint32_t>(M)) L.list[own][e] = static_cast<uint32_t>(start + j);
             }
             if (dens64 && d2 < tc)
-                atomicAdd(&L.pacc[own], static_cast<double>(dens_weight(kern, c.x, c.y, c.z, o.x, o.y, o.z, hc, d2)));
+                atomicAdd(&L.pacc[own], static_cast<double>(dens_weight(KERN, c.x, c.y, c.z, o.x, o.y, o.z, hc, d2)));
         }
     }
     wave_lds_fence();
@@ -677,13 +677,13 @@ __device__ __forceinline__ void collect_leaf(CollectLds<M>& L, int start, int co
 // entries than its list holds sends the wavefront to the insertion kernel,
 // which redoes the search and scatters the terms at tau only (everything
 // else is settled here).
-template <int K, int M, int P, bool kAll>
+template <int K, int M, int P, bool kAll, int KERN>
 __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
     int leaf_size, int init_leaves, const uint32_t* __restrict__ wleaf, const uint32_t* __restrict__ wleaf_n,
     const float* __restrict__ tau_sorted, float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted,
     double* __restrict__ dens64, double* __restrict__ dens_pull, int32_t* __restrict__ nbr_out,
-    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats, int kern)
+    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats)
 {
     __shared__ CollectLds<M> lds[kCollectWaves];
     const int lane = threadIdx.x % kWave;
@@ -763,8 +763,8 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
                 const uint64_t mask = (static_cast<uint64_t>(__builtin_amdgcn_readlane(m_hi, cur)) << 32) |
                                       static_cast<uint32_t>(__builtin_amdgcn_readlane(m_lo, cur));
                 ++n_leaf;
-                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64,
-                                         n_pairs, kern);
+                collect_leaf<M, P, kAll, KERN>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64,
+                                               n_pairs);
             }
         }
     } else {
@@ -783,8 +783,8 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
                 const int count = b.hi_w;
                 const float4 c = (j < count) ? spos[start + j] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
                 const float tc = (j < count) ? tau_sorted[start + j] : 0.0f;
-                collect_leaf<M, P, kAll>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64, n_pairs,
-                                         kern);
+                collect_leaf<M, P, kAll, KERN>(L, start, count, mask, c, tc, lane, qx, qy, qz, tau, hinv, dens64,
+                                               n_pairs);
             } else {
                 stack = (lane == sp) ? 2 * node + 1 : stack;
                 ++sp;
@@ -829,7 +829,7 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
             }
             if (r < K - 1 && dens64)
                 unsafeAtomicAdd(&dens64[si],
-                                static_cast<double>(dens_weight(kern, qx, qy, qz, c.x, c.y, c.z, hinv, d2c)));
+                                static_cast<double>(dens_weight(KERN, qx, qy, qz, c.x, c.y, c.z, hinv, d2c)));
             if (r == K - 1) kk = make_key(tau, idx);
         }
     }
@@ -1162,10 +1162,17 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     }
     mark_end(0);
     mark_begin(1);
-#define ASM_COLLECT(M, PAD, ALL)                                                                                \
-    knn_collect_kernel<kSlots, M, PAD, ALL><<<col_blocks, kCollectThreads, 0, s>>>(                             \
+    // The density kernel is a template argument of the collect walk: the
+    // default tent path carries no test of the kernel in its inner loop.
+#define ASM_COLLECT_K(M, PAD, ALL, KERN)                                                                        \
+    knn_collect_kernel<kSlots, M, PAD, ALL, KERN><<<col_blocks, kCollectThreads, 0, s>>>(                       \
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.wleaf, scratch.wleaf_n, scratch.tau,  \
-        rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats, scratch.kernel)
+        rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats)
+#define ASM_COLLECT(M, PAD, ALL)                                                                                \
+    do {                                                                                                        \
+        if (scratch.kernel == kKernSpline) ASM_COLLECT_K(M, PAD, ALL, kKernSpline);                             \
+        else ASM_COLLECT_K(M, PAD, ALL, kKernTent);                                                             \
+    } while (0)
 #define ASM_COLLECT_PAD(M, ALL)                  \
     if (leaf_size <= 8) ASM_COLLECT(M, 8, ALL);  \
     else if (leaf_size <= 16) ASM_COLLECT(M, 16, ALL); \
@@ -1178,6 +1185,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     }
 #undef ASM_COLLECT_PAD
 #undef ASM_COLLECT
+#undef ASM_COLLECT_K
     ASM_HIP_CHECK(hipGetLastError());
     mark_end(1);
     // Wavefronts flagged by the collect walk are redone by the insertion

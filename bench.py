@@ -96,9 +96,9 @@ def make_workload(n_side: int, L: float, bf: float, seed: int):
 def run_step(ctx, L: float, bf: float, ncs):
     ctx.construct(-bf * L, (1 + bf) * L)
     ctx.neighbor_search(32)
+    ctx.assign_on_meshes(ncs, -1)          # every nc of the list in one pass
     totals = []
     for nc in ncs:
-        ctx.assign_on_mesh(nc, -1)
         totals.append(ctx.mesh_total(nc))
         ctx.normalize_mesh(nc)
     return totals

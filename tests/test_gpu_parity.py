@@ -455,3 +455,24 @@ def test_spline_kernel_with_ties_matches_oracle(ctx, method):
             assert rel_err(dens, odens) <= DENS_RTOL, (keep, det)
     finally:
         ctx.set_option(A.OPT_KERNEL, 0)
+
+
+@pytest.mark.parametrize("ncs,tile", [((8, 16, 32), 8), ((1, 2, 3, 16), 5), ((12, 7, 12, 20), 3), ((24, 6), 8)])
+def test_all_meshes_in_one_pass_match_oracle(ctx, ncs, tile):
+    """asm_assign_on_meshes: every nc of the list from one count pass, one fill
+    pass and one tile launch over the meshes' concatenated tiles; each mesh
+    equals the oracle and the one-mesh path, whatever the order, repeats,
+    sizes (one to three cells per side) and tile side."""
+    xv, L, bf = small_set("blobs", 16, 31)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+    o = oracle_path(buf, L, sorted(set(ncs)))
+    g = gpu_path(ctx, parts, nl, L, bf, [], tile=tile)
+    ctx.assign_on_meshes(list(ncs), -1)
+    vmax = float(np.max(np.abs(buf[:, 3:])))
+    multi = {nc: ctx.get_mesh(nc) for nc in set(ncs)}
+    for nc in set(ncs):
+        _check_mesh(multi[nc], o["meshes"][nc], nc, vmax)
+        ctx.assign_on_mesh(nc, -1)
+        one = ctx.get_mesh(nc)
+        assert rel_err(multi[nc][: nc ** 3], one[: nc ** 3], floor=1e-30) <= MESH_RTOL
