@@ -2,7 +2,8 @@
 
 Runs, in one process: a plain device copy of a known number of bytes (the
 calibration), then one pass of the smoothing path on the 256^3 Zel'dovich
-workload (tree, kNN + density, deposition of nc = 64, 128, 256).  Dividing
+workload (tree, kNN + density, deposition of nc = 64, 128, 256 in one
+pass).  Dividing
 the counter reading of each smoothing kernel by the copy kernel's reading per
 copied byte gives that kernel's HBM bytes per launch.
 
@@ -183,8 +184,8 @@ def main() -> int:
         ctx.set_particles(parts, nl, L)
         ctx.construct(-bf * L, (1 + bf) * L)
         ctx.neighbor_search(32)
+        ctx.assign_on_meshes(ncs, -1)      # every nc in one pass, as a step does
         for nc in ncs:
-            ctx.assign_on_mesh(nc, -1)
             ctx.mesh_total(nc)
         ctx.synchronize()
     print(f"path: n_buf {len(parts)} n_local {nl} ncs {ncs}", flush=True)

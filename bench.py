@@ -311,15 +311,18 @@ def main(argv=None) -> int:
 
     # Roofline of the dominant kernel (largest total time in the timed steps).
     # Timers are HIP events on the library's stream around each kernel (the
-    # kNN kernels individually; deposit_tiles once per mesh).
+    # kNN kernels individually; deposit_tiles once per launch, which covers
+    # every mesh of the nc list).
+    dep_launches = max(timers.get("deposit_tiles", (0, 0))[1], 1) / args.steps
     alg = {
         "knn_bound": (BYTES_KNN_BOUND * nbuf, "knn_bound_kernel",
                       "bound walk: read x, 12 B/particle x N_buf"),
         "knn_collect": (BYTES_KNN * nbuf, "knn_collect_kernel",
                         "collect walk: read x, write h and dens, 20 B/particle x N_buf"),
-        "deposit_tiles": (sum(BYTES_DEPOSIT * nbuf + BYTES_CELL * nc ** 3 for nc in ncs) / len(ncs),
-                          "deposit_tile_packed_kernel",
-                          "tile deposition; 36 B/particle x N_buf + 24 B/cell x nc^3, mean over the nc list"),
+        "deposit_tiles": (sum(BYTES_DEPOSIT * nbuf + BYTES_CELL * nc ** 3 for nc in ncs) / dep_launches,
+                          "multi_tile_kernel",
+                          "tile deposition of every mesh of the nc list in one launch; "
+                          "36 B/particle x N_buf + 24 B/cell x nc^3, summed over the list"),
         "sort": (BYTES_TREE * nbuf, "rs_scatter", "Morton radix sort; 24 B/particle"),
     }
     dom = max((k for k in alg if timers.get(k, (0, 0))[1]), key=lambda k: timers[k][0])

@@ -224,8 +224,13 @@ def test_learned_seed_heldout_share_is_exact():
     _log(f"fallback rate {rate:.2e}; distance evaluations per query plain "
          f"{plain_work['dist_evals'] / len(parts):.1f}, seeded {seeded_work['dist_evals'] / len(parts):.1f}")
     assert rate < 0.01
-    # the model bounds the search: it never makes the bound walk evaluate more
-    assert seeded_work["dist_evals"] <= plain_work["dist_evals"] * 1.01
+    # The model bounds the search: it never makes the bound walk evaluate
+    # more.  Nor does it make it evaluate much less: the walk streams the
+    # union of the wavefront's 64 search spheres, which a bound trims only
+    # until t[32] settles (even each particle's exact tau as the seed leaves
+    # about 93% of the evaluations), so the gain is absent and stays so.
+    ratio = seeded_work["dist_evals"] / plain_work["dist_evals"]
+    assert 0.9 <= ratio <= 1.01, ratio
     # the seeded result is the oracle's on a sampled region
     buf = _as_buf(parts)
     center = (100.0, 300.0, 420.0)
