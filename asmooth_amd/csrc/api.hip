@@ -110,6 +110,7 @@ struct asm_context {
     DevBuf<double> dens64;
     int deterministic = 0;
     int knn_method = kKnnBoundCollect;
+    int knn_bound = kBoundSelect;   // ASM_OPT_KNN_BOUND
     int deposit_accum = kAccumFp64;
     int kernel = kKernTent;   // ASM_OPT_KERNEL
     DevBuf<uint32_t> knn_flags, knn_leaf_n, knn_leaves;   // per wavefront of 64 sorted particles
@@ -330,6 +331,11 @@ int asm_set_option(asm_context* ctx, int option, int value)
             return fail(ctx, ASM_ERR_ARG, "kNN method %d is not 0 or 1", value);
         ctx->knn_method = value;
         return ASM_OK;
+    case ASM_OPT_KNN_BOUND:
+        if (value != kBoundSelect && value != kBoundPacket)
+            return fail(ctx, ASM_ERR_ARG, "kNN bound walk %d is not 0 or 1", value);
+        ctx->knn_bound = value;
+        return ASM_OK;
     case ASM_OPT_DEPOSIT_ACCUM:
         if (value != kAccumFp64 && value != kAccumKahan32 && value != kAccumAtomic32)
             return fail(ctx, ASM_ERR_ARG, "deposition accumulation %d is not 0, 1 or 2", value);
@@ -454,7 +460,7 @@ int asm_neighbor_search(asm_context* ctx, int k)
     }
     KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p,
                     nullptr, nullptr, nullptr, ctx->leaf_size % kGroup == 0 ? ctx->groups.p : nullptr, 0,
-                    ctx->kernel};
+                    ctx->kernel, ctx->knn_bound};
     ctx->seed_used = false;
     if (bound_collect && ctx->seed_on) {
         HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));

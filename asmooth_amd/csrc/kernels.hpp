@@ -67,6 +67,8 @@ constexpr int kStatSlots = 1024;
 // particles); kKnnInsertion: the insertion kernel for every wavefront.
 constexpr int kKnnBoundCollect = 0;
 constexpr int kKnnInsertion = 1;
+constexpr int kBoundSelect = 0;
+constexpr int kBoundPacket = 1;
 constexpr int kLeafCap = 256;   // leaf ids recorded per wavefront by the bound walk
 // Device work buffers of the bound/collect method: per wavefront of 64
 // sorted particles, wleaf kLeafCap ids, wleaf_n and wave_flags one each;
@@ -94,6 +96,9 @@ struct KnnScratch {
     int seed_walk;
     // Density kernel of the scatter: kKernTent or kKernSpline.
     int kernel;
+    // Walk 1 without a learned bound: kBoundSelect (per-query selection,
+    // needs group boxes) or kBoundPacket (the packet walk).
+    int bound;
 };
 // Optional marks around the kernels of the bound/collect method (stage 0:
 // bound walk, 1: collect walk, 2: insertion kernel for flagged wavefronts).

@@ -23,6 +23,11 @@
 //     compares or selects, and a lane whose d is not below t[32] is left
 //     unchanged by the same instructions.  At the end t[32] = tau, the 33rd
 //     smallest squared distance over all particles.
+//     By default walk 1 is instead a selection (knn_select_kernel): after a
+//     per-lane bound from the seed range and the union of the leaves within
+//     it, the wavefront settles its 64 queries one at a time, all lanes on
+//     one query, and takes tau as the 33rd of the query's candidates within
+//     its bound, sorted across the wavefront.  Same tau; fewer instructions.
 //  2. collect: a second walk with tau fixed opens exactly the leaves whose
 //     box distance is within tau and appends every particle with d2 <= tau to
 //     a per-lane list in LDS: normally 33 entries, more only when distances
@@ -595,6 +600,571 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Walk 1 by selection (the default).  The packet walk above makes every lane
+// evaluate every candidate that some lane of its wavefront needs: about 53
+// times the 33 slots kept.  Here a wavefront still takes 64 consecutive
+// particles, but after two cheap wavefront-wide phases it settles them one
+// query at a time, all 64 lanes on one query:
+//
+//  A. Bound: every lane counts the squared distances from its query to the
+//     particles of the seed range in a byte histogram in LDS (64 bins of an
+//     eighth of an octave of d2, below the seed range's own box diagonal).
+//     The upper edge of the bin holding the 33rd count is a bound rA2 >= tau:
+//     at least 33 particles lie within it.
+//  B. Union: a walk of the tree with the bounds fixed lists every leaf whose
+//     box lies within rA2 of some lane, with the mask of the lanes it does
+//     (the ballot of the box test).
+//  C. Per query: the listed leaves with the query's bit set give their group
+//     boxes, those within the query's bound their particles, and the squared
+//     distances within the bound go to a list in LDS.  The list is cut back
+//     whenever it passes 64 entries: the first 64 are sorted across the
+//     wavefront (a 21-stage bitonic network on the bit patterns, which order
+//     non-negative floats), the smallest 33 are kept and the bound drops to
+//     the 33rd.  At the end the list is sorted once more and tau is its 33rd
+//     entry.  Every particle with d2 <= tau lies within every bound the query
+//     had, so it was listed: tau is the 33rd smallest squared distance over
+//     all particles, ties included, in the reference's FP32 arithmetic.
+//
+// The listed leaves outside the seed range are recorded for the collect walk
+// (they include every leaf within tau of some lane).  A wavefront whose
+// union would exceed kLeafCap leaves, or whose seed range holds fewer than
+// 33 particles, is flagged in `redo` and left to the packet walk.
+constexpr int kSelThreads = 256;
+constexpr int kSelWaves = kSelThreads / kWave;
+constexpr int kSelBins = 64;                     // byte counters per lane
+constexpr int kSelHistWords = kSelBins / 4;
+constexpr int kSelSeedMax = 255;                 // seed particles counted (a byte bin never overflows)
+constexpr int kSelBinShift = 20;                 // d2 bit pattern >> 20: exponent and 3 mantissa bits
+
+struct SelectLds {
+    union {
+        struct {
+            uint32_t hist[kSelHistWords][kWave];  // lane l's bins 4w..4w+3 in word [w][l]
+            float4 stage[kWave];
+        } a;
+        struct {
+            uint64_t umask[kLeafCap];  // lanes whose bound reaches union leaf e
+            uint32_t uleaf[kLeafCap];
+            union {
+                struct {
+                    union {
+                        uint32_t front0[kLeafCap];  // a frontier
+                        float4 qtab[kWave];         // the queries (x, y, z, bound), for the masks
+                    };
+                    uint32_t front1[kLeafCap];      // a frontier, then the candidate leaves
+                };                                  // B
+                struct {
+                    uint32_t slist[kLeafCap];     // the union leaves one query needs
+                    uint32_t glist[2 * kWave];    // its groups within the bound, one round
+                    uint32_t cand[2 * kWave];     // its squared distances within the bound
+                } c;                              // C
+            };
+        } b;
+    };
+};
+
+// Partner value of lane ^ X.  Patterns within a row of 16 lanes are DPP
+// moves, the others within 32 lanes ds_swizzle (no memory access), and the
+// mirror across the whole wavefront a ds_bpermute.
+template <int X>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int lane)
+{
+    const int s = static_cast<int>(v);
+    if constexpr (X == 1) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, s, 0xB1, 0xF, 0xF, true));
+    else if constexpr (X == 2) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, s, 0x4E, 0xF, 0xF, true));
+    else if constexpr (X == 3) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, s, 0x1B, 0xF, 0xF, true));
+    else if constexpr (X == 7) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, s, 0x141, 0xF, 0xF, true));
+    else if constexpr (X == 15) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, s, 0x140, 0xF, 0xF, true));
+    else if constexpr (X == 63) return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute((63 - lane) << 2, s));
+    else {
+        static_assert(X == 4 || X == 8 || X == 16 || X == 31, "xor pattern");
+        return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(s, 0x1f | (X << 10)));
+    }
+}
+
+// One compare-exchange stage: lanes with bit M clear keep the smaller value.
+template <int X, int M>
+__device__ __forceinline__ uint32_t cx_stage(uint32_t v, int lane)
+{
+    const uint32_t p = xor_lane<X>(v, lane);
+    return (lane & M) ? max(v, p) : min(v, p);
+}
+
+// Ascending sort of one value per lane over the wavefront: a bitonic
+// network whose first stage of each merge of size k compares mirrored lanes
+// (lane ^ (k - 1)), the rest lane ^ j; the smaller value always goes to the
+// lower lane.  Must be called with all 64 lanes active.
+__device__ __forceinline__ uint32_t wave_sort64(uint32_t v, int lane)
+{
+    v = cx_stage<1, 1>(v, lane);
+    v = cx_stage<3, 2>(v, lane);
+    v = cx_stage<1, 1>(v, lane);
+    v = cx_stage<7, 4>(v, lane);
+    v = cx_stage<2, 2>(v, lane);
+    v = cx_stage<1, 1>(v, lane);
+    v = cx_stage<15, 8>(v, lane);
+    v = cx_stage<4, 4>(v, lane);
+    v = cx_stage<2, 2>(v, lane);
+    v = cx_stage<1, 1>(v, lane);
+    v = cx_stage<31, 16>(v, lane);
+    v = cx_stage<8, 8>(v, lane);
+    v = cx_stage<4, 4>(v, lane);
+    v = cx_stage<2, 2>(v, lane);
+    v = cx_stage<1, 1>(v, lane);
+    v = cx_stage<63, 32>(v, lane);
+    v = cx_stage<16, 16>(v, lane);
+    v = cx_stage<8, 8>(v, lane);
+    v = cx_stage<4, 4>(v, lane);
+    v = cx_stage<2, 2>(v, lane);
+    v = cx_stage<1, 1>(v, lane);
+    return v;
+}
+
+// Lane l of a row of 16 reads lane l ^ M of the same row: DPP quad
+// permutations (xor 1, 2, 3), half-row mirror (xor 7), row mirror (xor 15),
+// and their compositions for xor 4 (7 then 3) and xor 8 (15 then 7).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+template <int M>
+__device__ __forceinline__ uint32_t row_xor(uint32_t v)
+{
+    if constexpr (M == 1) return dpp_mov<0xB1>(v);
+    else if constexpr (M == 2) return dpp_mov<0x4E>(v);
+    else if constexpr (M == 3) return dpp_mov<0x1B>(v);
+    else if constexpr (M == 7) return dpp_mov<0x141>(v);
+    else if constexpr (M == 15) return dpp_mov<0x140>(v);
+    else if constexpr (M == 4) return dpp_mov<0x1B>(dpp_mov<0x141>(v));
+    else {
+        static_assert(M == 8, "row xor pattern");
+        return dpp_mov<0x141>(dpp_mov<0x140>(v));
+    }
+}
+__device__ __forceinline__ void cx_reg(uint32_t& a, uint32_t& b)
+{
+    const uint32_t lo = min(a, b);
+    b = max(a, b);
+    a = lo;
+}
+// Compare-exchange of element 4l+v with element 4(l^M)+w, w = 3-v (FLIP,
+// the mirror stage of a merge) or v; the lane with bit LB clear keeps the
+// smaller value.
+template <int M, int LB, bool FLIP>
+__device__ __forceinline__ void cx_row(uint32_t (&x)[4], int l16)
+{
+    uint32_t p[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) p[v] = row_xor<M>(x[FLIP ? 3 - v : v]);
+    const bool lower = (l16 & LB) == 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) x[v] = lower ? min(x[v], p[v]) : max(x[v], p[v]);
+}
+// Ascending sort of 64 values held by a row of 16 lanes, element 4l+v in
+// register v of lane l: the mirror bitonic network, its stages between
+// elements of one lane done in registers, the others by DPP within the row.
+// Needs every lane of the row active.
+__device__ __forceinline__ void row_sort64(uint32_t (&x)[4], int l16)
+{
+    cx_reg(x[0], x[1]);
+    cx_reg(x[2], x[3]);
+    const auto in_lane = [&]() {   // half-cleaners of element bits 1 and 0
+        cx_reg(x[0], x[2]);
+        cx_reg(x[1], x[3]);
+        cx_reg(x[0], x[1]);
+        cx_reg(x[2], x[3]);
+    };
+    cx_reg(x[0], x[3]);
+    cx_reg(x[1], x[2]);
+    cx_reg(x[0], x[1]);
+    cx_reg(x[2], x[3]);
+    cx_row<1, 1, true>(x, l16);
+    in_lane();
+    cx_row<3, 2, true>(x, l16);
+    cx_row<1, 1, false>(x, l16);
+    in_lane();
+    cx_row<7, 4, true>(x, l16);
+    cx_row<2, 2, false>(x, l16);
+    cx_row<1, 1, false>(x, l16);
+    in_lane();
+    cx_row<15, 8, true>(x, l16);
+    cx_row<4, 4, false>(x, l16);
+    cx_row<2, 2, false>(x, l16);
+    cx_row<1, 1, false>(x, l16);
+    in_lane();
+}
+
+__device__ __forceinline__ int lane_rank(uint64_t mask)
+{
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
+}
+
+template <int K>
+__global__ __launch_bounds__(kSelThreads) void knn_select_kernel(
+    const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
+    int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
+    uint32_t* __restrict__ wleaf_n, uint32_t* __restrict__ redo, KnnStats* __restrict__ stats,
+    const Box* __restrict__ groups)
+{
+    static_assert(K <= kWave, "the selection keeps K entries in one wavefront");
+    __shared__ SelectLds lds[kSelWaves];
+    const int lane = threadIdx.x % kWave;
+    const int wid = wave_in_block();
+    SelectLds& S = lds[wid];
+    const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kSelWaves + wid) * kWave;
+    if (wave0 >= n) return;
+    const int64_t wave = wave0 / kWave;
+    // Particle, leaf and node indices fit in 32 bits (the launcher checks n).
+    const int ni = static_cast<int>(n);
+    const int w0 = static_cast<int>(wave0);
+    const int nlp = static_cast<int>(nleaf_pad);
+    const int q = w0 + lane;
+    const bool valid = q < ni;
+    // Lanes past the end repeat the last particle, so wavefront-wide box
+    // tests need no validity mask.
+    const float4 me = spos[valid ? q : ni - 1];
+    const int nvalid = ni - w0 < kWave ? ni - w0 : kWave;
+
+    const int last = (ni < w0 + kWave ? ni : w0 + kWave) - 1;
+    int l0 = w0 / leaf_size - init_leaves;
+    int l1 = last / leaf_size + init_leaves;
+    if (l0 < 0) l0 = 0;
+    if (l1 > static_cast<int>(nleaf) - 1) l1 = static_cast<int>(nleaf) - 1;
+    const int s0 = l0 * leaf_size;
+    int s1 = (l1 + 1) * leaf_size < ni ? (l1 + 1) * leaf_size : ni;
+    if (s1 - s0 > kSelSeedMax) s1 = s0 + kSelSeedMax;
+    if (s1 - s0 < K) {
+        if (lane == 0) redo[wave] = 1u;
+        return;
+    }
+
+    // ---- A: per-lane bound from the seed range --------------------------
+    float blo0 = INFINITY, blo1 = INFINITY, blo2 = INFINITY;
+    float bhi0 = -INFINITY, bhi1 = -INFINITY, bhi2 = -INFINITY;
+    for (int l = l0; l <= l1; ++l) {
+        const Box b = nodes[nlp + l];
+        blo0 = fminf(blo0, b.lo[0]);
+        blo1 = fminf(blo1, b.lo[1]);
+        blo2 = fminf(blo2, b.lo[2]);
+        bhi0 = fmaxf(bhi0, b.hi[0]);
+        bhi1 = fmaxf(bhi1, b.hi[1]);
+        bhi2 = fmaxf(bhi2, b.hi[2]);
+    }
+    // Every squared distance within the seed range is at most the box
+    // diagonal's (plus rounding, under one bin step): bins end at its bin + 1.
+    const float diag2 = norm2(bhi0, bhi1, bhi2, blo0, blo1, blo2);
+    const int base = static_cast<int>(__float_as_uint(diag2) >> kSelBinShift) + 1 - (kSelBins - 1);
+#pragma unroll
+    for (int w = 0; w < kSelHistWords; ++w) S.a.hist[w][lane] = 0u;
+    float dmax = 0.0f;
+    for (int b0 = s0; b0 < s1; b0 += kWave) {
+        const int m = s1 - b0 < kWave ? s1 - b0 : kWave;
+        S.a.stage[lane] = lane < m ? spos[b0 + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        wave_lds_fence();
+        for (int j = 0; j < m; ++j) {
+            const float4 c = S.a.stage[j];
+            const float d2 = norm2(me.x, me.y, me.z, c.x, c.y, c.z);
+            dmax = fmaxf(dmax, d2);
+            const int k = static_cast<int>(__float_as_uint(d2) >> kSelBinShift) - base;
+            const int bin = k < 0 ? 0 : (k > kSelBins - 1 ? kSelBins - 1 : k);
+            atomicAdd(&S.a.hist[bin >> 2][lane], 1u << ((bin & 3) * 8));
+        }
+        wave_lds_fence();
+    }
+    int cum = 0, bsel = kSelBins - 1;
+    bool found = false;
+#pragma unroll
+    for (int w = 0; w < kSelHistWords; ++w) {
+        const uint32_t h = S.a.hist[w][lane];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            cum += static_cast<int>((h >> (8 * t)) & 0xffu);
+            if (!found && cum >= K) {
+                found = true;
+                bsel = 4 * w + t;
+            }
+        }
+    }
+    // The top bin also holds anything clamped into it: its bound is the
+    // largest squared distance seen.
+    const float rA2 = bsel == kSelBins - 1 ? dmax
+                                           : __uint_as_float(static_cast<uint32_t>(base + bsel + 1) << kSelBinShift);
+    wave_lds_fence();   // the histogram's LDS is reused below
+
+    // ---- B: union of the leaves within some lane's bound ----------------
+    // Level by level from the root, one frontier node per lane.  A child is
+    // kept when its box is within the largest bound of the box of the
+    // wavefront's own leaves (which holds every query); at the leaf level
+    // each lane takes one candidate leaf and tests it against every query
+    // of the wavefront, giving the leaf's mask of lanes.  Every node of a
+    // level lies at the same depth of the complete tree.
+    float qlo0 = INFINITY, qlo1 = INFINITY, qlo2 = INFINITY;
+    float qhi0 = -INFINITY, qhi1 = -INFINITY, qhi2 = -INFINITY;
+    for (int l = w0 / leaf_size; l <= last / leaf_size; ++l) {
+        const Box b = nodes[nlp + l];
+        qlo0 = fminf(qlo0, b.lo[0]);
+        qlo1 = fminf(qlo1, b.lo[1]);
+        qlo2 = fminf(qlo2, b.lo[2]);
+        qhi0 = fmaxf(qhi0, b.hi[0]);
+        qhi1 = fmaxf(qhi1, b.hi[1]);
+        qhi2 = fmaxf(qhi2, b.hi[2]);
+    }
+    const float rmax2 = wave_max(rA2);
+    // Lower bound of norm2 between any point of b and any query (monotone
+    // rounding, as box_d2).
+    const auto near_queries = [&](const Box& b) {
+        const float g0 = fmaxf(fmaxf(b.lo[0] - qhi0, qlo0 - b.hi[0]), 0.0f);
+        const float g1 = fmaxf(fmaxf(b.lo[1] - qhi1, qlo1 - b.hi[1]), 0.0f);
+        const float g2 = fmaxf(fmaxf(b.lo[2] - qhi2, qlo2 - b.hi[2]), 0.0f);
+        float d = g0 * g0;
+        d = d + g1 * g1;
+        d = d + g2 * g2;
+        return d <= rmax2;
+    };
+    // The frontier alternates between front0 and front1, one level each;
+    // the first buffer is chosen by the parity of the number of levels so
+    // that the candidate leaves end in front1 and front0 is free for the
+    // queries.
+    const int levels = 31 - __builtin_clz(static_cast<uint32_t>(nlp));   // nlp is a power of two
+    uint32_t* front = (levels & 1) ? S.b.front0 : S.b.front1;
+    uint32_t* next = (levels & 1) ? S.b.front1 : S.b.front0;
+    int nu = 0;
+    uint32_t n_node = 0;
+    bool over = false;
+    int ncand = 0;   // candidate leaves (in `next` after the walk)
+    if (nlp == 1) {
+        // one-leaf tree: the root is the leaf
+        next = S.b.front1;
+        if (lane == 0) next[0] = 0u;
+        ncand = 1;
+    } else {
+        if (lane == 0) front[0] = 1u;
+        int nf = 1;
+        int level_first = 1;   // index of the first node of the frontier's level
+        wave_lds_fence();
+        while (nf > 0 && !over) {
+            const bool leaves = 2 * level_first >= nlp;
+            int nn = 0;
+            for (int fb = 0; fb < nf; fb += kWave) {
+                const int j = fb + lane;
+                const bool act = j < nf;
+                const int node = act ? static_cast<int>(front[j]) : 1;
+                n_node += static_cast<uint32_t>(__popcll(wave_ballot(act)));
+                const int c0 = 2 * node;
+                const Box b0 = nodes[c0];
+                const Box b1 = nodes[c0 + 1];
+                const bool k0 = act && near_queries(b0);
+                const bool k1 = act && near_queries(b1);
+                const uint64_t m0 = wave_ballot(k0);
+                const uint64_t m1 = wave_ballot(k1);
+                const int s0r = nn + lane_rank(m0);
+                const int s1r = nn + __popcll(m0) + lane_rank(m1);
+                const uint32_t base_id = leaves ? static_cast<uint32_t>(c0 - nlp) : static_cast<uint32_t>(c0);
+                if (k0 && s0r < kLeafCap) next[s0r] = base_id;
+                if (k1 && s1r < kLeafCap) next[s1r] = base_id + 1u;
+                nn += __popcll(m0) + __popcll(m1);
+            }
+            if (nn > kLeafCap) over = true;
+            wave_lds_fence();
+            if (leaves) {
+                ncand = nn;
+                break;
+            }
+            uint32_t* const tmp = front;
+            front = next;
+            next = tmp;
+            nf = nn;
+            level_first *= 2;
+        }
+    }
+    if (!over) {
+        // Masks of the candidate leaves: lane e takes leaf e of a batch and
+        // tests it against every query, read from LDS by broadcast.  The
+        // queries go where the frontier was (front is free once the last
+        // level has written the candidates to `next`; for the one-leaf tree
+        // nothing else is there).
+        float4* const qtab = (next == S.b.front1) ? S.b.qtab : nullptr;
+        if (qtab) {
+            qtab[lane] = make_float4(me.x, me.y, me.z, rA2);
+            wave_lds_fence();
+            for (int cb = 0; cb < ncand && !over; cb += kWave) {
+                const int e = cb + lane;
+                const bool act = e < ncand;
+                const int leaf = act ? static_cast<int>(next[e]) : 0;
+                const Box b = nodes[nlp + leaf];
+                uint32_t mlo = 0u, mhi = 0u;
+                for (int j = 0; j < 32; ++j) {
+                    const float4 t = qtab[j];
+                    mlo |= (box_d2(b, t.x, t.y, t.z) <= t.w) ? (1u << j) : 0u;
+                }
+                for (int j = 0; j < 32; ++j) {
+                    const float4 t = qtab[32 + j];
+                    mhi |= (box_d2(b, t.x, t.y, t.z) <= t.w) ? (1u << j) : 0u;
+                }
+                const uint64_t mk = (static_cast<uint64_t>(mhi) << 32) | mlo;
+                const bool add = act && mk != 0ull;
+                const uint64_t ab = wave_ballot(add);
+                const int slot = nu + lane_rank(ab);
+                if (add && slot < kLeafCap) {
+                    S.b.uleaf[slot] = static_cast<uint32_t>(leaf);
+                    S.b.umask[slot] = mk;
+                }
+                nu += __popcll(ab);
+                if (nu > kLeafCap) over = true;
+            }
+        } else {
+            over = true;   // not reached: the candidates always end in front1
+        }
+    }
+    if (over) {
+        if (lane == 0) redo[wave] = 1u;
+        return;
+    }
+    wave_lds_fence();
+
+    // ---- C: one query at a time -----------------------------------------
+    // Three stages per query, each issuing its loads together: the union
+    // leaves the query needs (from the masks), the boxes of their groups (up
+    // to 128 per round, two loads per lane), and the particles of the groups
+    // within the bound (up to 32 groups per round, four loads per lane),
+    // whose squared distances within the bound go to a list.  Whenever the
+    // list passes 64 entries the bound drops to the 33rd smallest of its
+    // first 64 (wave_select) and only those at or below it stay, 33 of them
+    // with ties; at the end tau is the 33rd smallest of the list.
+    const int gpl = leaf_size / kGroup;      // groups per leaf (leaf_size % kGroup == 0)
+    const int lpr = 2 * kWave / gpl;         // leaves per round of group boxes
+    int pr_leaf[2], pr_grp[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int pr = k * kWave + lane;     // (leaf, group) pair of the round
+        pr_leaf[k] = pr / gpl;
+        pr_grp[k] = pr - pr_leaf[k] * gpl;
+    }
+    uint32_t* const slist = S.b.c.slist;
+    uint32_t* const glist = S.b.c.glist;
+    uint32_t* const cand = S.b.c.cand;
+    uint32_t n_eval = 0, n_acc = 0;
+    float tau = 0.0f;
+    for (int i = 0; i < nvalid; ++i) {
+        float4 qq;
+        qq.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.x), i));
+        qq.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.y), i));
+        qq.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.z), i));
+        float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rA2), i));
+        const uint64_t bit = 1ull << i;
+        int ns = 0;
+        for (int ub = 0; ub < nu; ub += kWave) {
+            const int e = ub + lane;
+            const bool has = e < nu && (S.b.umask[e] & bit) != 0ull;
+            const uint64_t sb = wave_ballot(has);
+            if (has) slist[ns + lane_rank(sb)] = S.b.uleaf[e];
+            ns += __popcll(sb);
+        }
+        wave_lds_fence();
+        int cc = 0;
+        for (int t = 0; t < ns; t += lpr) {
+            Box gbx[2];
+            bool gv[2];
+            int gidx[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int r = t + pr_leaf[k];
+                gv[k] = pr_leaf[k] < lpr && r < ns;
+                gidx[k] = 0;
+                if (gv[k]) {
+                    const int p0 = static_cast<int>(slist[r]) * leaf_size + pr_grp[k] * kGroup;
+                    gv[k] = p0 < ni;
+                    gidx[k] = p0 / kGroup;
+                }
+                if (gv[k]) gbx[k] = groups[gidx[k]];
+            }
+            int ng = 0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const bool ga = gv[k] && box_d2(gbx[k], qq.x, qq.y, qq.z) <= r2;
+                const uint64_t gb = wave_ballot(ga);
+                if (ga) glist[ng + lane_rank(gb)] = static_cast<uint32_t>(gidx[k]);
+                ng += __popcll(gb);
+            }
+            wave_lds_fence();
+            constexpr int kGpb = kWave / kGroup;   // groups per batch of 64 particles
+            for (int u = 0; u < ng; u += 4 * kGpb) {
+                float4 cp[4];
+                bool pv[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int gi = u + k * kGpb + lane / kGroup;
+                    const int p = gi < ng ? static_cast<int>(glist[gi]) * kGroup + (lane % kGroup) : ni;
+                    pv[k] = p < ni;
+                    cp[k] = pv[k] ? spos[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (u + k * kGpb >= ng) break;
+                    const float d2 = norm2(qq.x, qq.y, qq.z, cp[k].x, cp[k].y, cp[k].z);
+                    const bool acc = pv[k] && d2 <= r2;
+                    const uint64_t ab = wave_ballot(acc);
+                    n_eval += static_cast<uint32_t>(__popcll(wave_ballot(pv[k])));
+                    if (acc) cand[cc + lane_rank(ab)] = __float_as_uint(d2);
+                    cc += __popcll(ab);
+                    n_acc += static_cast<uint32_t>(__popcll(ab));
+                    wave_lds_fence();
+                    while (cc > kWave) {
+                        // Bound to the 33rd smallest of the first 64; keep
+                        // those below it and enough at it to make 33, then
+                        // the rest (at most 64) at or below it.
+                        uint32_t x = cand[lane];
+                        const bool more = kWave + lane < cc;
+                        const uint32_t xe = more ? cand[kWave + lane] : 0xffffffffu;
+                        x = wave_sort64(x, lane);
+                        const uint32_t kb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), K - 1));
+                        r2 = __uint_as_float(kb);
+                        const bool keep_e = more && xe <= kb;
+                        const uint64_t ke = wave_ballot(keep_e);
+                        if (lane < K) cand[lane] = x;
+                        if (keep_e) cand[K + lane_rank(ke)] = xe;
+                        cc = K + __popcll(ke);
+                        wave_lds_fence();
+                    }
+                }
+            }
+        }
+        uint32_t x = lane < cc ? cand[lane] : 0xffffffffu;
+        x = wave_sort64(x, lane);
+        const float ti = __uint_as_float(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), K - 1)));
+        tau = (lane == i) ? ti : tau;
+        wave_lds_fence();
+    }
+    if (valid) tau_out[q] = tau;
+
+    // Leaves for the collect walk: the union outside the seed range.
+    uint32_t* const my_leaves = wleaf + wave * kLeafCap;
+    int n_rec = 0;
+    for (int ub = 0; ub < nu; ub += kWave) {
+        const int e = ub + lane;
+        const int leaf = e < nu ? static_cast<int>(S.b.uleaf[e]) : 0;
+        const bool rec = e < nu && (leaf < l0 || leaf > l1);
+        const uint64_t rb = wave_ballot(rec);
+        if (rec) my_leaves[n_rec + lane_rank(rb)] = static_cast<uint32_t>(leaf);
+        n_rec += __popcll(rb);
+    }
+    if (lane == 0) {
+        wleaf_n[wave] = static_cast<uint32_t>(n_rec);
+        redo[wave] = 0u;
+    }
+    if (stats && lane == 0) {
+        KnnStats* const st = stats + (wave & (kStatSlots - 1));
+        atomicAdd(&st->node_tests, static_cast<unsigned long long>(n_node));
+        atomicAdd(&st->leaf_scans, static_cast<unsigned long long>(nu));
+        atomicAdd(&st->dist_evals, static_cast<unsigned long long>(s1 - s0) * static_cast<unsigned long long>(nvalid) +
+                                       static_cast<unsigned long long>(n_eval));
+        atomicAdd(&st->inserts, static_cast<unsigned long long>(n_acc));
+    }
+}
+
 // LDS of one wavefront in the collect walk.
 template <int M>
 struct CollectLds {
@@ -1147,6 +1717,18 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
         else if (leaf_size <= 32) ASM_SEED_WALK(32);
         else ASM_SEED_WALK(64);
 #undef ASM_SEED_WALK
+    } else if (!scratch.tau_init && scratch.bound == kBoundSelect && groups) {
+        const unsigned sel_blocks = static_cast<unsigned>((waves + kSelWaves - 1) / kSelWaves);
+        knn_select_kernel<kSlots><<<sel_blocks, kSelThreads, 0, s>>>(
+            spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
+            scratch.wave_flags, stats, groups);
+        ASM_HIP_CHECK(hipGetLastError());
+        // Wavefronts the selection flagged (union past kLeafCap leaves, or a
+        // seed range under 33 particles): the packet walk; every other
+        // wavefront returns at once.  The collect walk rewrites the flags.
+        knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
+            spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
+            stats, nullptr, scratch.wave_flags, nullptr, nullptr, groups);
     } else {
         knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
             spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,

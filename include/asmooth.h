@@ -76,13 +76,19 @@ enum {
                                       the plain search from the learned bounds; 1 searches within
                                       them by a range walk (a study: slower on the 256^3
                                       workload).  Results are the same. */
-    ASM_OPT_KERNEL = 9          /* smoothing kernel: 0 (default) the reference's separable tent
+    ASM_OPT_KERNEL = 9,         /* smoothing kernel: 0 (default) the reference's separable tent
                                       (ker, nbr_search.cpp:27-37; coarse_mesh.cpp:10-22); 1 the
                                       cubic spline: spline_ker (nbr_search.cpp:39-53, which the
                                       reference defines but does not call) for the density, and
                                       the separable cubic B-spline of half-width h, integrated
                                       exactly over each cell, on the mesh.  Setting it requires
                                       a new asm_neighbor_search. */
+    ASM_OPT_KNN_BOUND = 10      /* first walk of ASM_OPT_KNN_METHOD 0 (the 33rd distance of
+                                      every particle): 0 (default) selection, one query at a
+                                      time over its wavefront with a bitonic sort of the
+                                      candidates within a seed-range bound; 1 the packet walk
+                                      (64 queries share every candidate, med3 insertion).
+                                      Same results.  With a seed model the packet walk runs. */
 };
 
 const char* asm_version(void);

@@ -10,9 +10,9 @@
   Kahan and FP32 atomic cell sums within their bounds of the same oracle
   cells.
 - The held-out 512^3 set (seed 200), one node share of a 2^3 split, with the
-  trained seed model: seeded h bit-exact against the plain search over the
-  whole share, the fallback rate below its bound, and the seeded h tied to
-  the oracle on a sampled region.
+  trained seed model: seeded h bit-exact against the plain search (selection
+  walk and packet walk) over the whole share, the fallback rate below its
+  bound, and the seeded h tied to the oracle on a sampled region.
 
 Sets too large for the oracle whole are checked by region_oracle
 (tests/helpers.py), whose own assertions prove that the sampled particles'
@@ -210,7 +210,15 @@ def test_learned_seed_heldout_share_is_exact():
         c.construct(-bf * L, (1 + bf) * L)
         c.neighbor_search(32)
         rk, dens, _ = c.get_rk_dens()
+        # the seed bounds the packet walk: its work against the unseeded
+        # packet walk's (the default selection walk does less work anyway)
+        c.set_option(A.OPT_KNN_BOUND, 1)
+        c.construct(-bf * L, (1 + bf) * L)
+        c.neighbor_search(32)
+        prk, _, _ = c.get_rk_dens()
+        assert np.array_equal(prk.view(np.int32), rk.view(np.int32))
         plain_work = c.search_stats()
+        c.set_option(A.OPT_KNN_BOUND, 0)
         c.set_seed_model(model, 0.3)
         c.construct(-bf * L, (1 + bf) * L)
         c.neighbor_search(32)

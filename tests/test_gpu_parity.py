@@ -140,11 +140,12 @@ def test_small_trees_match_oracle(ctx, n, leaf):
     assert rel_err(dens, O.density(pos, onbr, ork)) <= DENS_RTOL
 
 
-def _knn_run(ctx, pos, method, deterministic=0, keep=1):
+def _knn_run(ctx, pos, method, deterministic=0, keep=1, bound=0):
     parts = A.particles_from_xv(np.concatenate([pos, np.zeros_like(pos)], 1))
     ctx.set_option(A.OPT_KEEP_NEIGHBORS, keep)
     ctx.set_option(A.OPT_KNN_METHOD, method)
     ctx.set_option(A.OPT_DETERMINISTIC, deterministic)
+    ctx.set_option(A.OPT_KNN_BOUND, bound)
     try:
         lo, hi = float(pos.min()) - 0.5, float(pos.max()) + 0.5
         ctx.set_particles(parts, len(pos), hi - lo)
@@ -155,6 +156,7 @@ def _knn_run(ctx, pos, method, deterministic=0, keep=1):
     finally:
         ctx.set_option(A.OPT_KNN_METHOD, 0)
         ctx.set_option(A.OPT_DETERMINISTIC, 0)
+        ctx.set_option(A.OPT_KNN_BOUND, 0)
 
 
 def _tie_heavy_sets():
@@ -202,6 +204,53 @@ def test_knn_ties_match_oracle(ctx, name, method):
         _, dens_g, _ = _knn_run(ctx, pos, method, deterministic=1, keep=keep)
         assert np.array_equal(np.isnan(dens_g), nan)
         assert rel_err(dens_g[~nan], odens[~nan]) <= DENS_RTOL
+
+
+@pytest.mark.parametrize("name", ["quantized30", "lattice7", "coincident50"])
+@pytest.mark.parametrize("leaf", [16, 24, 32])
+@pytest.mark.parametrize("bound", [0, 1])
+def test_bound_walks_on_ties_match_oracle(ctx, name, leaf, bound):
+    """Walk 1 by selection (ASM_OPT_KNN_BOUND 0: per-query bitonic sort of the
+    candidates within a seed-range bound) and by the packet walk (1), at leaf
+    sizes with 2, 3 and 4 group boxes per leaf, on the tie-heavy sets: h and
+    the 33-slot lists bit-exact against the oracle."""
+    pos = _tie_heavy_sets()[name]
+    onbr, _, ork = O.knn(pos)
+    ctx.set_option(A.OPT_LEAF_SIZE, leaf)
+    try:
+        rk, dens, nbr = _knn_run(ctx, pos, 0, bound=bound)
+    finally:
+        ctx.set_option(A.OPT_LEAF_SIZE, 32)
+    assert np.array_equal(rk.view(np.int32), ork.view(np.int32))
+    assert np.array_equal(nbr, onbr)
+
+
+@pytest.mark.parametrize("kind,n_side,seed", [("zeldovich", 20, 4), ("blobs", 20, 6), ("uniform", 16, 8)])
+def test_bound_walks_agree(ctx, kind, n_side, seed):
+    xv, L, bf = small_set(kind, n_side, seed)
+    parts, nl = A.build_buffered_set(xv, L, bf)
+    pos = np.ascontiguousarray(parts["x"])
+    r0 = _knn_run(ctx, pos, 0, bound=0)
+    r1 = _knn_run(ctx, pos, 0, bound=1)
+    assert np.array_equal(r0[0].view(np.int32), r1[0].view(np.int32))
+    assert np.array_equal(r0[2], r1[2])
+    assert rel_err(r0[1], r1[1]) <= DENS_RTOL
+
+
+def test_selection_union_overflow_falls_back(ctx):
+    """Sparse particles around a dense blob of 20,000: wavefronts of sparse
+    particles have seed-range bounds that reach the whole blob (over 600
+    leaves), more than the selection lists (256), and are left to the packet
+    walk; every h and list is still the oracle's."""
+    rng = np.random.default_rng(21)
+    blob = rng.normal(50.0, 1.0, (20000, 3))
+    sparse = rng.uniform(0.0, 100.0, (3000, 3))
+    pos = np.concatenate([blob, sparse]).astype(np.float32)
+    onbr, _, ork = O.knn(pos)
+    rk, dens, nbr = _knn_run(ctx, pos, 0, bound=0)
+    assert np.array_equal(rk.view(np.int32), ork.view(np.int32))
+    assert np.array_equal(nbr, onbr)
+    assert rel_err(dens, O.density(pos, onbr, ork)) <= DENS_RTOL
 
 
 @pytest.mark.parametrize("kind,n_side,seed", [("zeldovich", 20, 4), ("blobs", 20, 6)])
