@@ -114,6 +114,7 @@ struct asm_context {
     int deposit_accum = kAccumFp64;
     int kernel = kKernTent;   // ASM_OPT_KERNEL
     DevBuf<uint32_t> knn_flags, knn_leaf_n, knn_leaves;   // per wavefront of 64 sorted particles
+    DevBuf<uint32_t> knn_list, knn_list_n;               // selection walk's neighbour lists
     DevBuf<float> knn_tau;                                // per sorted particle
     DevBuf<double> dens_pull;
     // learned seed radius
@@ -458,9 +459,17 @@ int asm_neighbor_search(asm_context* ctx, int k)
         HIPC(ctx->knn_tau.ensure(n));
         if (!ctx->deterministic) HIPC(ctx->dens_pull.ensure(n));
     }
+    // Lists of the selection walk (not with kept ordered lists, which the
+    // collect walk writes)
+    const bool sel_lists = bound_collect && ctx->knn_bound == kBoundSelect && !ctx->keep_neighbors;
+    if (sel_lists) {
+        HIPC(ctx->knn_list.ensure(nwaves * kWave * kListCap));
+        HIPC(ctx->knn_list_n.ensure(n));
+    }
     KnnScratch kscr{ctx->knn_leaves.p, ctx->knn_leaf_n.p, ctx->knn_flags.p, ctx->knn_tau.p, ctx->dens_pull.p,
                     nullptr, nullptr, nullptr, ctx->leaf_size % kGroup == 0 ? ctx->groups.p : nullptr, 0,
-                    ctx->kernel, ctx->knn_bound};
+                    ctx->kernel, ctx->knn_bound, sel_lists ? ctx->knn_list.p : nullptr,
+                    sel_lists ? ctx->knn_list_n.p : nullptr};
     ctx->seed_used = false;
     if (bound_collect && ctx->seed_on) {
         HIPC(ctx->seed_grid.ensure(static_cast<size_t>(kMlpGrid) * kMlpGrid * kMlpGrid));

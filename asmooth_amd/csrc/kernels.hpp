@@ -69,7 +69,11 @@ constexpr int kKnnBoundCollect = 0;
 constexpr int kKnnInsertion = 1;
 constexpr int kBoundSelect = 0;
 constexpr int kBoundPacket = 1;
-constexpr int kLeafCap = 256;   // leaf ids recorded per wavefront by the bound walk
+constexpr int kLeafCap = 256;
+// Per-particle neighbour lists of the selection walk (every particle within
+// tau: 33 plus ties), kListCap sorted positions per particle; a particle with
+// more ties sends its wavefront to the collect walk.
+constexpr int kListCap = 40;   // a multiple of 4 (blocks of four entries per lane)   // leaf ids recorded per wavefront by the bound walk
 // Device work buffers of the bound/collect method: per wavefront of 64
 // sorted particles, wleaf kLeafCap ids, wleaf_n and wave_flags one each;
 // per sorted particle, tau (33rd squared distance) and dens_pull (density
@@ -99,6 +103,14 @@ struct KnnScratch {
     // Walk 1 without a learned bound: kBoundSelect (per-query selection,
     // needs group boxes) or kBoundPacket (the packet walk).
     int bound;
+    // With the selection walk and no kept lists: per particle kListCap
+    // sorted positions (per wavefront, blocks of four entries per lane) and
+    // the list length.  The density is
+    // then settled from the lists, and the collect walk runs only for the
+    // wavefronts the selection could not list.  Null: the collect walk for
+    // every wavefront.
+    uint32_t* list;
+    uint32_t* list_n;
 };
 // Optional marks around the kernels of the bound/collect method (stage 0:
 // bound walk, 1: collect walk, 2: insertion kernel for flagged wavefronts).

@@ -648,17 +648,15 @@ struct SelectLds {
             uint32_t uleaf[kLeafCap];
             union {
                 struct {
-                    union {
-                        uint32_t front0[kLeafCap];  // a frontier
-                        float4 qtab[kWave];         // the queries (x, y, z, bound), for the masks
-                    };
-                    uint32_t front1[kLeafCap];      // a frontier, then the candidate leaves
-                };                                  // B
+                    uint32_t front0[kLeafCap];  // a frontier
+                    uint32_t front1[kLeafCap];  // a frontier, or the kept leaves of a batch
+                };                              // B
                 struct {
-                    uint32_t slist[kLeafCap];     // the union leaves one query needs
-                    uint32_t glist[2 * kWave];    // its groups within the bound, one round
-                    uint32_t cand[2 * kWave];     // its squared distances within the bound
-                } c;                              // C
+                    uint32_t slist[2 * kWave];  // union leaves one query needs, 128 union entries at a time
+                    uint32_t glist[2 * kWave];  // their groups within its bound, one round
+                    uint32_t cand[2 * kWave];   // its squared distances within the bound ...
+                    uint32_t cpos[2 * kWave];   // ... and their sorted positions
+                } c;                            // C
             };
         } b;
     };
@@ -796,6 +794,13 @@ __device__ __forceinline__ void row_sort64(uint32_t (&x)[4], int l16)
     in_lane();
 }
 
+// Entry r of the list of lane l of wavefront w: blocks of four entries per
+// lane, the lanes' blocks side by side (a wavefront reads 1 KB per block).
+__device__ __forceinline__ int64_t list_slot(int64_t w, int r, int l)
+{
+    return ((w * (kListCap / 4) + (r >> 2)) * kWave + l) * 4 + (r & 3);
+}
+
 __device__ __forceinline__ int lane_rank(uint64_t mask)
 {
     return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
@@ -803,11 +808,12 @@ __device__ __forceinline__ int lane_rank(uint64_t mask)
 }
 
 template <int K>
-__global__ __launch_bounds__(kSelThreads) void knn_select_kernel(
+__global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))) void knn_select_kernel(
     const float4* __restrict__ spos, int64_t n, const Box* __restrict__ nodes, int64_t nleaf_pad, int64_t nleaf,
     int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
     uint32_t* __restrict__ wleaf_n, uint32_t* __restrict__ redo, KnnStats* __restrict__ stats,
-    const Box* __restrict__ groups)
+    const Box* __restrict__ groups, uint32_t* __restrict__ list, uint32_t* __restrict__ list_n,
+    float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted)
 {
     static_assert(K <= kWave, "the selection keeps K entries in one wavefront");
     __shared__ SelectLds lds[kSelWaves];
@@ -924,23 +930,55 @@ __global__ __launch_bounds__(kSelThreads) void knn_select_kernel(
         d = d + g2 * g2;
         return d <= rmax2;
     };
-    // The frontier alternates between front0 and front1, one level each;
-    // the first buffer is chosen by the parity of the number of levels so
-    // that the candidate leaves end in front1 and front0 is free for the
-    // queries.
-    const int levels = 31 - __builtin_clz(static_cast<uint32_t>(nlp));   // nlp is a power of two
-    uint32_t* front = (levels & 1) ? S.b.front0 : S.b.front1;
-    uint32_t* next = (levels & 1) ? S.b.front1 : S.b.front0;
+    // Each lane's mask of the queries whose bound reaches box b.
+    const auto query_mask = [&](const Box& b) {
+        uint32_t mlo = 0u, mhi = 0u;
+        for (int j = 0; j < 32; ++j) {
+            const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.x), j));
+            const float y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.y), j));
+            const float z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.z), j));
+            const float rb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rA2), j));
+            mlo |= (box_d2(b, x, y, z) <= rb) ? (1u << j) : 0u;
+        }
+        for (int j = 0; j < 32; ++j) {
+            const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.x), 32 + j));
+            const float y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.y), 32 + j));
+            const float z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.z), 32 + j));
+            const float rb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rA2), 32 + j));
+            mhi |= (box_d2(b, x, y, z) <= rb) ? (1u << j) : 0u;
+        }
+        return (static_cast<uint64_t>(mhi) << 32) | mlo;
+    };
     int nu = 0;
     uint32_t n_node = 0;
     bool over = false;
-    int ncand = 0;   // candidate leaves (in `next` after the walk)
+    // Masks of `cnt` kept leaves listed in `kept`: lane e takes leaf e and
+    // tests it against every query; leaves some query needs join the union.
+    const auto add_leaves = [&](const uint32_t* kept, int cnt) {
+        for (int cb = 0; cb < cnt; cb += kWave) {
+            const int e = cb + lane;
+            const bool act = e < cnt;
+            const int leaf = act ? static_cast<int>(kept[e]) : 0;
+            const uint64_t mk = query_mask(nodes[nlp + leaf]);
+            const bool add = act && mk != 0ull;
+            const uint64_t ab = wave_ballot(add);
+            const int slot = nu + lane_rank(ab);
+            if (add && slot < kLeafCap) {
+                S.b.uleaf[slot] = static_cast<uint32_t>(leaf);
+                S.b.umask[slot] = mk;
+            }
+            nu += __popcll(ab);
+        }
+        if (nu > kLeafCap) over = true;
+    };
     if (nlp == 1) {
         // one-leaf tree: the root is the leaf
-        next = S.b.front1;
-        if (lane == 0) next[0] = 0u;
-        ncand = 1;
+        if (lane == 0) S.b.front1[0] = 0u;
+        wave_lds_fence();
+        add_leaves(S.b.front1, 1);
     } else {
+        uint32_t* front = S.b.front0;
+        uint32_t* next = S.b.front1;
         if (lane == 0) front[0] = 1u;
         int nf = 1;
         int level_first = 1;   // index of the first node of the frontier's level
@@ -948,7 +986,7 @@ __global__ __launch_bounds__(kSelThreads) void knn_select_kernel(
         while (nf > 0 && !over) {
             const bool leaves = 2 * level_first >= nlp;
             int nn = 0;
-            for (int fb = 0; fb < nf; fb += kWave) {
+            for (int fb = 0; fb < nf && !over; fb += kWave) {
                 const int j = fb + lane;
                 const bool act = j < nf;
                 const int node = act ? static_cast<int>(front[j]) : 1;
@@ -960,63 +998,31 @@ __global__ __launch_bounds__(kSelThreads) void knn_select_kernel(
                 const bool k1 = act && near_queries(b1);
                 const uint64_t m0 = wave_ballot(k0);
                 const uint64_t m1 = wave_ballot(k1);
-                const int s0r = nn + lane_rank(m0);
-                const int s1r = nn + __popcll(m0) + lane_rank(m1);
-                const uint32_t base_id = leaves ? static_cast<uint32_t>(c0 - nlp) : static_cast<uint32_t>(c0);
-                if (k0 && s0r < kLeafCap) next[s0r] = base_id;
-                if (k1 && s1r < kLeafCap) next[s1r] = base_id + 1u;
-                nn += __popcll(m0) + __popcll(m1);
+                if (leaves) {
+                    // the batch's kept leaves (at most 128), then their masks
+                    const int s0r = lane_rank(m0);
+                    const int s1r = __popcll(m0) + lane_rank(m1);
+                    if (k0) next[s0r] = static_cast<uint32_t>(c0 - nlp);
+                    if (k1) next[s1r] = static_cast<uint32_t>(c0 + 1 - nlp);
+                    wave_lds_fence();
+                    add_leaves(next, __popcll(m0) + __popcll(m1));
+                    wave_lds_fence();
+                } else {
+                    const int s0r = nn + lane_rank(m0);
+                    const int s1r = nn + __popcll(m0) + lane_rank(m1);
+                    if (k0 && s0r < kLeafCap) next[s0r] = static_cast<uint32_t>(c0);
+                    if (k1 && s1r < kLeafCap) next[s1r] = static_cast<uint32_t>(c0 + 1);
+                    nn += __popcll(m0) + __popcll(m1);
+                }
             }
             if (nn > kLeafCap) over = true;
             wave_lds_fence();
-            if (leaves) {
-                ncand = nn;
-                break;
-            }
+            if (leaves) break;
             uint32_t* const tmp = front;
             front = next;
             next = tmp;
             nf = nn;
             level_first *= 2;
-        }
-    }
-    if (!over) {
-        // Masks of the candidate leaves: lane e takes leaf e of a batch and
-        // tests it against every query, read from LDS by broadcast.  The
-        // queries go where the frontier was (front is free once the last
-        // level has written the candidates to `next`; for the one-leaf tree
-        // nothing else is there).
-        float4* const qtab = (next == S.b.front1) ? S.b.qtab : nullptr;
-        if (qtab) {
-            qtab[lane] = make_float4(me.x, me.y, me.z, rA2);
-            wave_lds_fence();
-            for (int cb = 0; cb < ncand && !over; cb += kWave) {
-                const int e = cb + lane;
-                const bool act = e < ncand;
-                const int leaf = act ? static_cast<int>(next[e]) : 0;
-                const Box b = nodes[nlp + leaf];
-                uint32_t mlo = 0u, mhi = 0u;
-                for (int j = 0; j < 32; ++j) {
-                    const float4 t = qtab[j];
-                    mlo |= (box_d2(b, t.x, t.y, t.z) <= t.w) ? (1u << j) : 0u;
-                }
-                for (int j = 0; j < 32; ++j) {
-                    const float4 t = qtab[32 + j];
-                    mhi |= (box_d2(b, t.x, t.y, t.z) <= t.w) ? (1u << j) : 0u;
-                }
-                const uint64_t mk = (static_cast<uint64_t>(mhi) << 32) | mlo;
-                const bool add = act && mk != 0ull;
-                const uint64_t ab = wave_ballot(add);
-                const int slot = nu + lane_rank(ab);
-                if (add && slot < kLeafCap) {
-                    S.b.uleaf[slot] = static_cast<uint32_t>(leaf);
-                    S.b.umask[slot] = mk;
-                }
-                nu += __popcll(ab);
-                if (nu > kLeafCap) over = true;
-            }
-        } else {
-            over = true;   // not reached: the candidates always end in front1
         }
     }
     if (over) {
@@ -1046,8 +1052,11 @@ __global__ __launch_bounds__(kSelThreads) void knn_select_kernel(
     uint32_t* const slist = S.b.c.slist;
     uint32_t* const glist = S.b.c.glist;
     uint32_t* const cand = S.b.c.cand;
+    uint32_t* const cpos = S.b.c.cpos;
     uint32_t n_eval = 0, n_acc = 0;
     float tau = 0.0f;
+    uint32_t my_n = 0;          // with lists: lane i's list length ...
+    uint32_t my_p33 = 0;        // ... and the sorted position of its 33rd key's particle
     for (int i = 0; i < nvalid; ++i) {
         float4 qq;
         qq.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.x), i));
@@ -1055,90 +1064,166 @@ __global__ __launch_bounds__(kSelThreads) void knn_select_kernel(
         qq.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.z), i));
         float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rA2), i));
         const uint64_t bit = 1ull << i;
-        int ns = 0;
-        for (int ub = 0; ub < nu; ub += kWave) {
-            const int e = ub + lane;
-            const bool has = e < nu && (S.b.umask[e] & bit) != 0ull;
-            const uint64_t sb = wave_ballot(has);
-            if (has) slist[ns + lane_rank(sb)] = S.b.uleaf[e];
-            ns += __popcll(sb);
-        }
-        wave_lds_fence();
         int cc = 0;
-        for (int t = 0; t < ns; t += lpr) {
-            Box gbx[2];
-            bool gv[2];
-            int gidx[2];
+        for (int uc = 0; uc < nu; uc += 2 * kWave) {
+            int ns = 0;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int r = t + pr_leaf[k];
-                gv[k] = pr_leaf[k] < lpr && r < ns;
-                gidx[k] = 0;
-                if (gv[k]) {
-                    const int p0 = static_cast<int>(slist[r]) * leaf_size + pr_grp[k] * kGroup;
-                    gv[k] = p0 < ni;
-                    gidx[k] = p0 / kGroup;
-                }
-                if (gv[k]) gbx[k] = groups[gidx[k]];
-            }
-            int ng = 0;
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const bool ga = gv[k] && box_d2(gbx[k], qq.x, qq.y, qq.z) <= r2;
-                const uint64_t gb = wave_ballot(ga);
-                if (ga) glist[ng + lane_rank(gb)] = static_cast<uint32_t>(gidx[k]);
-                ng += __popcll(gb);
+            for (int h = 0; h < 2; ++h) {
+                const int e = uc + h * kWave + lane;
+                const bool has = e < nu && (S.b.umask[e] & bit) != 0ull;
+                const uint64_t sb = wave_ballot(has);
+                if (has) slist[ns + lane_rank(sb)] = S.b.uleaf[e];
+                ns += __popcll(sb);
             }
             wave_lds_fence();
-            constexpr int kGpb = kWave / kGroup;   // groups per batch of 64 particles
-            for (int u = 0; u < ng; u += 4 * kGpb) {
-                float4 cp[4];
-                bool pv[4];
+            for (int t = 0; t < ns; t += lpr) {
+                Box gbx[2];
+                bool gv[2];
+                int gidx[2];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int gi = u + k * kGpb + lane / kGroup;
-                    const int p = gi < ng ? static_cast<int>(glist[gi]) * kGroup + (lane % kGroup) : ni;
-                    pv[k] = p < ni;
-                    cp[k] = pv[k] ? spos[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                for (int k = 0; k < 2; ++k) {
+                    const int r = t + pr_leaf[k];
+                    gv[k] = pr_leaf[k] < lpr && r < ns;
+                    gidx[k] = 0;
+                    if (gv[k]) {
+                        const int p0 = static_cast<int>(slist[r]) * leaf_size + pr_grp[k] * kGroup;
+                        gv[k] = p0 < ni;
+                        gidx[k] = p0 / kGroup;
+                    }
+                    if (gv[k]) gbx[k] = groups[gidx[k]];
                 }
+                int ng = 0;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (u + k * kGpb >= ng) break;
-                    const float d2 = norm2(qq.x, qq.y, qq.z, cp[k].x, cp[k].y, cp[k].z);
-                    const bool acc = pv[k] && d2 <= r2;
-                    const uint64_t ab = wave_ballot(acc);
-                    n_eval += static_cast<uint32_t>(__popcll(wave_ballot(pv[k])));
-                    if (acc) cand[cc + lane_rank(ab)] = __float_as_uint(d2);
-                    cc += __popcll(ab);
-                    n_acc += static_cast<uint32_t>(__popcll(ab));
-                    wave_lds_fence();
-                    while (cc > kWave) {
-                        // Bound to the 33rd smallest of the first 64; keep
-                        // those below it and enough at it to make 33, then
-                        // the rest (at most 64) at or below it.
-                        uint32_t x = cand[lane];
-                        const bool more = kWave + lane < cc;
-                        const uint32_t xe = more ? cand[kWave + lane] : 0xffffffffu;
-                        x = wave_sort64(x, lane);
-                        const uint32_t kb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), K - 1));
-                        r2 = __uint_as_float(kb);
-                        const bool keep_e = more && xe <= kb;
-                        const uint64_t ke = wave_ballot(keep_e);
-                        if (lane < K) cand[lane] = x;
-                        if (keep_e) cand[K + lane_rank(ke)] = xe;
-                        cc = K + __popcll(ke);
+                for (int k = 0; k < 2; ++k) {
+                    const bool ga = gv[k] && box_d2(gbx[k], qq.x, qq.y, qq.z) <= r2;
+                    const uint64_t gb = wave_ballot(ga);
+                    if (ga) glist[ng + lane_rank(gb)] = static_cast<uint32_t>(gidx[k]);
+                    ng += __popcll(gb);
+                }
+                wave_lds_fence();
+                constexpr int kGpb = kWave / kGroup;   // groups per batch of 64 particles
+                for (int u = 0; u < ng; u += 4 * kGpb) {
+                    float4 cp[4];
+                    bool pv[4];
+                    int p_of[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int gi = u + k * kGpb + lane / kGroup;
+                        const int p = gi < ng ? static_cast<int>(glist[gi]) * kGroup + (lane % kGroup) : ni;
+                        pv[k] = p < ni;
+                        p_of[k] = p;
+                        cp[k] = pv[k] ? spos[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (u + k * kGpb >= ng) break;
+                        const float d2 = norm2(qq.x, qq.y, qq.z, cp[k].x, cp[k].y, cp[k].z);
+                        const bool acc = pv[k] && d2 <= r2;
+                        const uint64_t ab = wave_ballot(acc);
+                        n_eval += static_cast<uint32_t>(__popcll(wave_ballot(pv[k])));
+                        if (acc) {
+                            const int slot = cc + lane_rank(ab);
+                            cand[slot] = __float_as_uint(d2);
+                            cpos[slot] = static_cast<uint32_t>(p_of[k]);
+                        }
+                        cc += __popcll(ab);
+                        n_acc += static_cast<uint32_t>(__popcll(ab));
                         wave_lds_fence();
+                        while (cc > kWave) {
+                            // Bound to the 33rd smallest of the first 64.  Keep
+                            // those below it and, at it, all of them when lists
+                            // are written (the list of a query holds every
+                            // particle within its tau), else enough to make 33;
+                            // then the rest (at most 64) at or below it; again
+                            // while more than 64 remain.  A pass that keeps
+                            // everything (ties at the bound filling the list)
+                            // sends the wavefront to the packet walk.
+                            const uint32_t x = cand[lane];
+                            const uint32_t xp = cpos[lane];
+                            const bool more = kWave + lane < cc;
+                            const uint32_t xe = more ? cand[kWave + lane] : 0xffffffffu;
+                            const uint32_t xep = more ? cpos[kWave + lane] : 0u;
+                            const uint32_t kb = static_cast<uint32_t>(
+                                __builtin_amdgcn_readlane(static_cast<int>(wave_sort64(x, lane)), K - 1));
+                            r2 = __uint_as_float(kb);
+                            const int n_lt = __popcll(wave_ballot(x < kb));
+                            const uint64_t eq = wave_ballot(x == kb);
+                            const bool keep_x = x < kb || (x == kb && (list || lane_rank(eq) < K - n_lt));
+                            const uint64_t kx = wave_ballot(keep_x);
+                            const bool keep_e = more && xe <= kb;
+                            const uint64_t ke = wave_ballot(keep_e);
+                            const int nk = __popcll(kx) + __popcll(ke);
+                            if (nk >= cc) {
+                                if (lane == 0) redo[wave] = 1u;
+                                return;
+                            }
+                            if (keep_x) {
+                                cand[lane_rank(kx)] = x;
+                                cpos[lane_rank(kx)] = xp;
+                            }
+                            if (keep_e) {
+                                cand[__popcll(kx) + lane_rank(ke)] = xe;
+                                cpos[__popcll(kx) + lane_rank(ke)] = xep;
+                            }
+                            cc = nk;
+                            wave_lds_fence();
+                        }
                     }
                 }
             }
         }
-        uint32_t x = lane < cc ? cand[lane] : 0xffffffffu;
-        x = wave_sort64(x, lane);
-        const float ti = __uint_as_float(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), K - 1)));
+        const uint32_t x = lane < cc ? cand[lane] : 0xffffffffu;
+        const uint32_t tb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wave_sort64(x, lane)), K - 1));
+        const float ti = __uint_as_float(tb);
         tau = (lane == i) ? ti : tau;
+        if (list) {
+            // The query's list: every candidate at or below tau (the
+            // collect walk's list), in any order; its 33rd key: tau and the
+            // (33 - n_lt)-th smallest original index among the entries at tau.
+            const bool in = lane < cc && x <= tb;
+            const uint64_t im = wave_ballot(in);
+            const int n_in = __popcll(im);
+            if (n_in > kListCap) {
+                if (lane == 0) redo[wave] = 1u;
+                return;
+            }
+            const uint32_t xp = cpos[lane];
+            if (in) {
+                const int r = lane_rank(im);
+                list[list_slot(wave, r, i)] = xp;
+            }
+            const bool at = lane < cc && x == tb;
+            const uint64_t am = wave_ballot(at);
+            uint32_t p33;   // sorted position of the 33rd key's particle
+            if ((am & (am - 1)) == 0ull) {
+                // one entry at tau: it is the 33rd
+                p33 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(xp), __builtin_ctzll(am)));
+            } else {
+                // ties at tau: the (33 - n_lt)-th smallest original index
+                const int n_lt = __popcll(wave_ballot(lane < cc && x < tb));
+                const uint32_t orig = at ? __float_as_uint(spos[xp].w) : 0xffffffffu;
+                int rank = 0;
+                for (uint64_t m = am; m; m &= m - 1) {
+                    const uint32_t o =
+                        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(orig), __builtin_ctzll(m)));
+                    rank += o < orig ? 1 : 0;
+                }
+                const uint64_t sel = wave_ballot(at && rank == K - 1 - n_lt);
+                p33 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(xp), __builtin_ctzll(sel)));
+            }
+            my_n = (lane == i) ? static_cast<uint32_t>(n_in) : my_n;
+            my_p33 = (lane == i) ? p33 : my_p33;
+        }
         wave_lds_fence();
     }
-    if (valid) tau_out[q] = tau;
+    if (valid) {
+        tau_out[q] = tau;
+        if (list) {
+            list_n[q] = my_n;
+            rk_sorted[q] = sqrtf(tau);
+            kkey_sorted[q] = make_key(tau, __float_as_uint(spos[my_p33].w));
+        }
+    }
 
     // Leaves for the collect walk: the union outside the seed range.
     uint32_t* const my_leaves = wleaf + wave * kLeafCap;
@@ -1253,7 +1338,7 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
     int leaf_size, int init_leaves, const uint32_t* __restrict__ wleaf, const uint32_t* __restrict__ wleaf_n,
     const float* __restrict__ tau_sorted, float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted,
     double* __restrict__ dens64, double* __restrict__ dens_pull, int32_t* __restrict__ nbr_out,
-    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats)
+    uint32_t* __restrict__ wave_flags, KnnStats* __restrict__ stats, bool only_flagged)
 {
     __shared__ CollectLds<M> lds[kCollectWaves];
     const int lane = threadIdx.x % kWave;
@@ -1262,6 +1347,9 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
     const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kCollectWaves + wid) * kWave;
     if (wave0 >= n) return;
     const int64_t wave = wave0 / kWave;
+    // With only_flagged, the wavefronts whose flag is clear were settled
+    // from the selection walk's lists.
+    if (only_flagged && wave_flags[wave] == 0u) return;
     const uint32_t n_rec = wleaf_n[wave];
     const int64_t q = wave0 + lane;
     const bool valid = q < n;
@@ -1417,6 +1505,48 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
             if (r < K) nbr_out[oq * K + r] = static_cast<int32_t>(__float_as_uint(c.w));
         }
     }
+}
+
+// Density from the selection walk's lists (wavefronts whose flag is clear;
+// the collect walk takes the others).  Lane q takes its list: every particle
+// c with d2 <= tau_q.  The terms are settled as in collect_leaf, with q as the
+// owner: c's term on q is pulled when d2 < tau_c; q's term on c is scattered
+// with an FP64 atomic when d2 < tau_q and d2 > tau_c (c does not list q), and
+// at d2 = tau_q when c's key is below q's 33rd key (a tie in q's first 32).
+constexpr int kListThreads = 256;
+
+template <int KERN>
+__global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
+    const float4* __restrict__ spos, int64_t n, const float* __restrict__ tau_sorted,
+    const uint64_t* __restrict__ kkey_sorted, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_n,
+    const uint32_t* __restrict__ wave_flags, double* __restrict__ dens64, double* __restrict__ dens_pull)
+{
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * kListThreads + threadIdx.x;
+    if (q >= n) return;
+    const int64_t wave = q / kWave;
+    if (wave_flags[wave] != 0u) return;
+    const float4 me = spos[q];
+    const float tau = tau_sorted[q];
+    const float hinv = 1.0f / sqrtf(tau);
+    const uint64_t kk = kkey_sorted[q];
+    const uint32_t cnt = list_n[q];
+    double pacc = 0.0;
+    const int lane = static_cast<int>(q % kWave);
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t j = list[list_slot(wave, static_cast<int>(k), lane)];
+        const float4 c = spos[j];
+        const float tc = tau_sorted[j];
+        const float d2 = norm2(me.x, me.y, me.z, c.x, c.y, c.z);
+        if (d2 < tc) {
+            const float hc = 1.0f / sqrtf(tc);   // c's 1/h, as c's own lane computes it
+            pacc += static_cast<double>(dens_weight(KERN, c.x, c.y, c.z, me.x, me.y, me.z, hc, d2));
+        }
+        const bool give = d2 < tau ? d2 > tc : make_key(d2, __float_as_uint(c.w)) < kk;
+        if (give)
+            unsafeAtomicAdd(&dens64[j],
+                            static_cast<double>(dens_weight(KERN, me.x, me.y, me.z, c.x, c.y, c.z, hinv, d2)));
+    }
+    dens_pull[q] = pacc;
 }
 
 // Walk 1 with a learned bound, as a range walk.  Each lane's learned bound
@@ -1706,6 +1836,10 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     const Box* const groups = (leaf_size % kGroup == 0) ? scratch.groups : nullptr;
     const unsigned bound_blocks = static_cast<unsigned>((waves + kBoundWaves - 1) / kBoundWaves);
     const unsigned col_blocks = static_cast<unsigned>((waves + kCollectWaves - 1) / kCollectWaves);
+    // The selection walk lists each particle's neighbours when no ordered
+    // lists are kept; the collect walk then runs only where it could not.
+    const bool lists = !scratch.tau_init && scratch.bound == kBoundSelect && groups && scratch.list &&
+                       scratch.list_n && !nbr_out_orig;
     mark_begin(0);
     if (scratch.tau_init && scratch.seed_walk) {
 #define ASM_SEED_WALK(PAD)                                                                                        \
@@ -1721,7 +1855,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
         const unsigned sel_blocks = static_cast<unsigned>((waves + kSelWaves - 1) / kSelWaves);
         knn_select_kernel<kSlots><<<sel_blocks, kSelThreads, 0, s>>>(
             spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
-            scratch.wave_flags, stats, groups);
+            scratch.wave_flags, stats, groups, lists ? scratch.list : nullptr, scratch.list_n, rk_sorted, kkey_sorted);
         ASM_HIP_CHECK(hipGetLastError());
         // Wavefronts the selection flagged (union past kLeafCap leaves, or a
         // seed range under 33 particles): the packet walk; every other
@@ -1744,12 +1878,24 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     }
     mark_end(0);
     mark_begin(1);
+    if (lists && dens64) {
+        const unsigned lb = static_cast<unsigned>((n + kListThreads - 1) / kListThreads);
+        if (scratch.kernel == kKernSpline)
+            knn_list_dens_kernel<kKernSpline><<<lb, kListThreads, 0, s>>>(spos, n, scratch.tau, kkey_sorted, scratch.list,
+                                                                         scratch.list_n, scratch.wave_flags, dens64,
+                                                                         scratch.dens_pull);
+        else
+            knn_list_dens_kernel<kKernTent><<<lb, kListThreads, 0, s>>>(spos, n, scratch.tau, kkey_sorted, scratch.list,
+                                                                       scratch.list_n, scratch.wave_flags, dens64,
+                                                                       scratch.dens_pull);
+        ASM_HIP_CHECK(hipGetLastError());
+    }
     // The density kernel is a template argument of the collect walk: the
     // default tent path carries no test of the kernel in its inner loop.
 #define ASM_COLLECT_K(M, PAD, ALL, KERN)                                                                        \
     knn_collect_kernel<kSlots, M, PAD, ALL, KERN><<<col_blocks, kCollectThreads, 0, s>>>(                       \
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.wleaf, scratch.wleaf_n, scratch.tau,  \
-        rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats)
+        rk_sorted, kkey_sorted, dens64, scratch.dens_pull, nbr_out_orig, scratch.wave_flags, stats, lists)
 #define ASM_COLLECT(M, PAD, ALL)                                                                                \
     do {                                                                                                        \
         if (scratch.kernel == kKernSpline) ASM_COLLECT_K(M, PAD, ALL, kKernSpline);                             \

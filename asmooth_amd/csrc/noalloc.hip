@@ -167,7 +167,7 @@ extern "C" int asm_smooth_noalloc(int device, const asm_particle* d_particles, i
     NAC(launch_gather_sorted(reinterpret_cast<const ParticleRec*>(d_particles), perm, n, spos, sattr, inv_perm, s));
     NAC(launch_build_tree(spos, n, kLeaf, nleaf, nleaf_pad, nodes, groups, s));
     // Search, h and density (asm_neighbor_search, bound + collect walks).
-    KnnScratch kscr{wleaf, wleaf_n, wave_flags, tau, dens_pull, nullptr, nullptr, nullptr, groups, 0, kKernTent, kBoundSelect};
+    KnnScratch kscr{wleaf, wleaf_n, wave_flags, tau, dens_pull, nullptr, nullptr, nullptr, groups, 0, kKernTent, kBoundSelect, nullptr, nullptr};
     NAC(launch_fill_zero_f64(dens64, n, s));
     NAC(launch_knn(spos, n, nodes, nleaf_pad, nleaf, kLeaf, rk, kkey, inv_perm, dens64, nullptr, nullptr, kscr,
                    kKnnBoundCollect, s));
