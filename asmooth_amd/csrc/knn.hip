@@ -608,8 +608,9 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
 // query at a time, all 64 lanes on one query:
 //
 //  A. Bound: every lane counts the squared distances from its query to the
-//     particles of the seed range in a byte histogram in LDS (64 bins of an
-//     eighth of an octave of d2, below the seed range's own box diagonal).
+//     particles of its seed (the wavefront's particles and 95 on either side
+//     in Morton order) in a byte histogram in LDS (64 bins of an eighth of
+//     an octave of d2, below the seed's own box diagonal).
 //     The upper edge of the bin holding the 33rd count is a bound rA2 >= tau:
 //     at least 33 particles lie within it.
 //  B. Union: a walk of the tree with the bounds fixed lists every leaf whose
@@ -628,13 +629,14 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
 //
 // The listed leaves outside the seed range are recorded for the collect walk
 // (they include every leaf within tau of some lane).  A wavefront whose
-// union would exceed kLeafCap leaves, or whose seed range holds fewer than
-// 33 particles, is flagged in `redo` and left to the packet walk.
+// union would exceed kLeafCap leaves, or whose seed holds fewer than 33
+// particles, is flagged in `redo` and left to the packet walk.
 constexpr int kSelThreads = 256;
 constexpr int kSelWaves = kSelThreads / kWave;
 constexpr int kSelBins = 64;                     // byte counters per lane
 constexpr int kSelHistWords = kSelBins / 4;
-constexpr int kSelSeedMax = 255;                 // seed particles counted (a byte bin never overflows)
+constexpr int kSelSeedHalo = 95;                 // seed particles on either side of the wavefront's 64
+static_assert(kWave + 2 * kSelSeedHalo <= 255, "a byte bin never overflows");
 constexpr int kSelBinShift = 20;                 // d2 bit pattern >> 20: exponent and 3 mantissa bits
 
 struct SelectLds {
@@ -839,9 +841,10 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     int l1 = last / leaf_size + init_leaves;
     if (l0 < 0) l0 = 0;
     if (l1 > static_cast<int>(nleaf) - 1) l1 = static_cast<int>(nleaf) - 1;
-    const int s0 = l0 * leaf_size;
-    int s1 = (l1 + 1) * leaf_size < ni ? (l1 + 1) * leaf_size : ni;
-    if (s1 - s0 > kSelSeedMax) s1 = s0 + kSelSeedMax;
+    // Seed of the bound: the wavefront's particles and kSelSeedHalo on
+    // either side in Morton order (at most 254 particles).
+    const int s0 = w0 - kSelSeedHalo > 0 ? w0 - kSelSeedHalo : 0;
+    const int s1 = w0 + kWave + kSelSeedHalo < ni ? w0 + kWave + kSelSeedHalo : ni;
     if (s1 - s0 < K) {
         if (lane == 0) redo[wave] = 1u;
         return;
@@ -850,7 +853,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     // ---- A: per-lane bound from the seed range --------------------------
     float blo0 = INFINITY, blo1 = INFINITY, blo2 = INFINITY;
     float bhi0 = -INFINITY, bhi1 = -INFINITY, bhi2 = -INFINITY;
-    for (int l = l0; l <= l1; ++l) {
+    for (int l = s0 / leaf_size; l <= (s1 - 1) / leaf_size; ++l) {
         const Box b = nodes[nlp + l];
         blo0 = fminf(blo0, b.lo[0]);
         blo1 = fminf(blo1, b.lo[1]);
@@ -859,7 +862,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
         bhi1 = fmaxf(bhi1, b.hi[1]);
         bhi2 = fmaxf(bhi2, b.hi[2]);
     }
-    // Every squared distance within the seed range is at most the box
+    // Every squared distance within the seed is at most the box
     // diagonal's (plus rounding, under one bin step): bins end at its bin + 1.
     const float diag2 = norm2(bhi0, bhi1, bhi2, blo0, blo1, blo2);
     const int base = static_cast<int>(__float_as_uint(diag2) >> kSelBinShift) + 1 - (kSelBins - 1);
@@ -1008,6 +1011,29 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
                     add_leaves(next, __popcll(m0) + __popcll(m1));
                     wave_lds_fence();
                 } else {
+                    const int s0r = nn + lane_rank(m0);
+                    const int s1r = nn + __popcll(m0) + lane_rank(m1);
+                    if (k0 && s0r < kLeafCap) next[s0r] = static_cast<uint32_t>(c0);
+                    if (k1 && s1r < kLeafCap) next[s1r] = static_cast<uint32_t>(c0 + 1);
+                    nn += __popcll(m0) + __popcll(m1);
+                }
+            }
+            if (!leaves && nn > kLeafCap) {
+                // The box test against all queries at once kept too many:
+                // the level again, keeping only nodes some query's own
+                // bound reaches.
+                nn = 0;
+                for (int fb = 0; fb < nf; fb += kWave) {
+                    const int j = fb + lane;
+                    const bool act = j < nf;
+                    const int node = act ? static_cast<int>(front[j]) : 1;
+                    const int c0 = 2 * node;
+                    const Box b0 = nodes[c0];
+                    const Box b1 = nodes[c0 + 1];
+                    const bool k0 = act && near_queries(b0) && query_mask(b0) != 0ull;
+                    const bool k1 = act && near_queries(b1) && query_mask(b1) != 0ull;
+                    const uint64_t m0 = wave_ballot(k0);
+                    const uint64_t m1 = wave_ballot(k1);
                     const int s0r = nn + lane_rank(m0);
                     const int s1r = nn + __popcll(m0) + lane_rank(m1);
                     if (k0 && s0r < kLeafCap) next[s0r] = static_cast<uint32_t>(c0);

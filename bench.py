@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes (compulsory FP32 traffic) per unit of work:
 BYTES_TREE = 24      # per particle: read x, write permuted x
 BYTES_KNN = 20       # per particle: read x, write h, write dens
-BYTES_KNN_BOUND = 12  # per particle: read x (the bound walk's compulsory input)
+BYTES_KNN_BOUND = 16  # per particle: read x, write tau (walk 1's compulsory input and output)
 BYTES_DEPOSIT = 36   # per particle per mesh: read x, v, h, dens, igrp
 BYTES_CELL = 24      # per cell per mesh: 6 fields written once
 
@@ -315,10 +315,12 @@ def main(argv=None) -> int:
     # every mesh of the nc list).
     dep_launches = max(timers.get("deposit_tiles", (0, 0))[1], 1) / args.steps
     alg = {
-        "knn_bound": (BYTES_KNN_BOUND * nbuf, "knn_bound_kernel",
-                      "bound walk: read x, 12 B/particle x N_buf"),
-        "knn_collect": (BYTES_KNN * nbuf, "knn_collect_kernel",
-                        "collect walk: read x, write h and dens, 20 B/particle x N_buf"),
+        "knn_bound": (BYTES_KNN_BOUND * nbuf, "knn_select_kernel",
+                      "walk 1 by selection (stage timer: the selection kernel and the packet walk for the "
+                      "wavefronts it hands over); read x, write tau: 16 B/particle x N_buf"),
+        "knn_collect": (BYTES_KNN * nbuf, "knn_list_dens_kernel",
+                        "density from the selection's lists (stage timer: with the collect walk for the "
+                        "wavefronts not listed); read x, write h and dens: 20 B/particle x N_buf"),
         "deposit_tiles": (sum(BYTES_DEPOSIT * nbuf + BYTES_CELL * nc ** 3 for nc in ncs) / dep_launches,
                           "multi_tile_kernel",
                           "tile deposition of every mesh of the nc list in one launch; "
