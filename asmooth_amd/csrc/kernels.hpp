@@ -73,7 +73,7 @@ constexpr int kLeafCap = 256;
 // Per-particle neighbour lists of the selection walk (every particle within
 // tau: 33 plus ties), kListCap sorted positions per particle; a particle with
 // more ties sends its wavefront to the collect walk.
-constexpr int kListCap = 40;   // a multiple of 4 (blocks of four entries per lane)   // leaf ids recorded per wavefront by the bound walk
+constexpr int kListCap = 40;   // a multiple of 4 (rows read 16 bytes at a time)   // leaf ids recorded per wavefront by the bound walk
 // Device work buffers of the bound/collect method: per wavefront of 64
 // sorted particles, wleaf kLeafCap ids, wleaf_n and wave_flags one each;
 // per sorted particle, tau (33rd squared distance) and dens_pull (density
@@ -103,9 +103,8 @@ struct KnnScratch {
     // Walk 1 without a learned bound: kBoundSelect (per-query selection,
     // needs group boxes) or kBoundPacket (the packet walk).
     int bound;
-    // With the selection walk and no kept lists: per particle kListCap
-    // sorted positions (per wavefront, blocks of four entries per lane) and
-    // the list length.  The density is
+    // With the selection walk and no kept lists: per particle a row of
+    // kListCap sorted positions and the list length.  The density is
     // then settled from the lists, and the collect walk runs only for the
     // wavefronts the selection could not list.  Null: the collect walk for
     // every wavefront.

@@ -796,11 +796,11 @@ __device__ __forceinline__ void row_sort64(uint32_t (&x)[4], int l16)
     in_lane();
 }
 
-// Entry r of the list of lane l of wavefront w: blocks of four entries per
-// lane, the lanes' blocks side by side (a wavefront reads 1 KB per block).
-__device__ __forceinline__ int64_t list_slot(int64_t w, int r, int l)
+// Entry r of the list of sorted particle q: one row of kListCap per
+// particle, so that a query's list is one contiguous store.
+__device__ __forceinline__ int64_t list_slot(int64_t q, int r)
 {
-    return ((w * (kListCap / 4) + (r >> 2)) * kWave + l) * 4 + (r & 3);
+    return q * kListCap + r;
 }
 
 __device__ __forceinline__ int lane_rank(uint64_t mask)
@@ -1216,7 +1216,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
             const uint32_t xp = cpos[lane];
             if (in) {
                 const int r = lane_rank(im);
-                list[list_slot(wave, r, i)] = xp;
+                list[list_slot(static_cast<int64_t>(w0) + i, r)] = xp;
             }
             const bool at = lane < cc && x == tb;
             const uint64_t am = wave_ballot(at);
@@ -1534,12 +1534,15 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
 }
 
 // Density from the selection walk's lists (wavefronts whose flag is clear;
-// the collect walk takes the others).  Lane q takes its list: every particle
-// c with d2 <= tau_q.  The terms are settled as in collect_leaf, with q as the
-// owner: c's term on q is pulled when d2 < tau_c; q's term on c is scattered
-// with an FP64 atomic when d2 < tau_q and d2 > tau_c (c does not list q), and
-// at d2 = tau_q when c's key is below q's 33rd key (a tie in q's first 32).
+// the collect walk takes the others).  Each particle q has a row of 16 lanes
+// (four particles per wavefront), lane l taking entries l, l + 16, l + 32 of
+// q's list: every particle c with d2 <= tau_q.  The terms are settled as in
+// collect_leaf, with q as the owner: c's term on q is pulled when d2 < tau_c
+// (summed over the row in FP64); q's term on c is scattered with an FP64
+// atomic when d2 < tau_q and d2 > tau_c (c does not list q), and at
+// d2 = tau_q when c's key is below q's 33rd key (a tie in q's first 32).
 constexpr int kListThreads = 256;
+constexpr int kListRow = 16;
 
 template <int KERN>
 __global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
@@ -1547,19 +1550,23 @@ __global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
     const uint64_t* __restrict__ kkey_sorted, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_n,
     const uint32_t* __restrict__ wave_flags, double* __restrict__ dens64, double* __restrict__ dens_pull)
 {
-    const int64_t q = static_cast<int64_t>(blockIdx.x) * kListThreads + threadIdx.x;
-    if (q >= n) return;
-    const int64_t wave = q / kWave;
-    if (wave_flags[wave] != 0u) return;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kListThreads + threadIdx.x;
+    const int64_t q = t / kListRow;
+    const int l16 = static_cast<int>(t % kListRow);
+    // A row (16 lanes) shares q: rows past the end and rows of flagged
+    // wavefronts leave together, so the row sum below runs on whole rows.
+    if (q >= n || wave_flags[q / kWave] != 0u) return;
     const float4 me = spos[q];
     const float tau = tau_sorted[q];
     const float hinv = 1.0f / sqrtf(tau);
     const uint64_t kk = kkey_sorted[q];
-    const uint32_t cnt = list_n[q];
+    const int cnt = static_cast<int>(list_n[q]);
     double pacc = 0.0;
-    const int lane = static_cast<int>(q % kWave);
-    for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t j = list[list_slot(wave, static_cast<int>(k), lane)];
+#pragma unroll
+    for (int e0 = 0; e0 < kListCap; e0 += kListRow) {
+        const int e = e0 + l16;
+        if (e >= cnt) break;
+        const uint32_t j = list[list_slot(q, e)];
         const float4 c = spos[j];
         const float tc = tau_sorted[j];
         const float d2 = norm2(me.x, me.y, me.z, c.x, c.y, c.z);
@@ -1569,10 +1576,11 @@ __global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
         }
         const bool give = d2 < tau ? d2 > tc : make_key(d2, __float_as_uint(c.w)) < kk;
         if (give)
-            unsafeAtomicAdd(&dens64[j],
-                            static_cast<double>(dens_weight(KERN, me.x, me.y, me.z, c.x, c.y, c.z, hinv, d2)));
+            unsafeAtomicAdd(&dens64[j], static_cast<double>(dens_weight(KERN, me.x, me.y, me.z, c.x, c.y, c.z, hinv, d2)));
     }
-    dens_pull[q] = pacc;
+#pragma unroll
+    for (int o = kListRow / 2; o > 0; o >>= 1) pacc += __shfl_xor(pacc, o, kListRow);
+    if (l16 == 0) dens_pull[q] = pacc;
 }
 
 // Walk 1 with a learned bound, as a range walk.  Each lane's learned bound
@@ -1905,7 +1913,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     mark_end(0);
     mark_begin(1);
     if (lists && dens64) {
-        const unsigned lb = static_cast<unsigned>((n + kListThreads - 1) / kListThreads);
+        const unsigned lb = static_cast<unsigned>((n * kListRow + kListThreads - 1) / kListThreads);
         if (scratch.kernel == kKernSpline)
             knn_list_dens_kernel<kKernSpline><<<lb, kListThreads, 0, s>>>(spos, n, scratch.tau, kkey_sorted, scratch.list,
                                                                          scratch.list_n, scratch.wave_flags, dens64,
