@@ -51,6 +51,7 @@ OPT_DEPOSIT_ACCUM = 7
 OPT_SEED_WALK = 8
 OPT_KERNEL = 9
 OPT_KNN_BOUND = 10
+OPT_BATCH_ORDER = 11
 KERNEL_TENT = 0
 KERNEL_SPLINE = 1
 # Limits of asm_smooth_noalloc (include/asmooth.h).

@@ -1146,6 +1146,7 @@ struct asm_batch {
     DevBuf<double> totals;          // per (cube, nc): sum n, sum halo, sum n^2, sum c (raw)
     DevBuf<uint32_t> overflow;      // per cube: tile lists overflowed in the last run
     std::vector<size_t> total_off;  // first totals slot of each cube
+    int order = 0;                  // ASM_OPT_BATCH_ORDER
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1272,6 +1273,11 @@ int asm_batch_set_option(asm_batch* b, int option, int value)
 {
     if (!b) return ASM_ERR_ARG;
     batch_release_graph(b);
+    if (option == ASM_OPT_BATCH_ORDER) {
+        if (value != 0 && value != 1) return batch_fail(b, ASM_ERR_ARG, "batch order %d is not 0 or 1", value);
+        b->order = value;
+        return ASM_OK;
+    }
     for (auto& c : b->cubes) {
         const int rc = asm_set_option(c.ctx, option, value);
         if (rc) {
@@ -1363,16 +1369,37 @@ int asm_batch_capture(asm_batch* b, int n_streams)
         return batch_fail(b, ASM_ERR_HIP, "%s", hipGetErrorString(cb));
     }
     hipError_t e = hipEventRecord(ev[ns], b->main);
-    // Shares on stream s = i % ns run in order on that stream.
+    // Which stream runs which share, and in what order: order 0 puts share i
+    // on stream i % ns in the order added; order 1 takes the shares largest
+    // first, each onto the stream with the fewest particles so far.
+    const size_t nc = b->cubes.size();
+    std::vector<size_t> order(nc);
+    std::vector<int> stream_of(nc);
+    for (size_t i = 0; i < nc; ++i) order[i] = i;
+    if (b->order == 1) {
+        std::stable_sort(order.begin(), order.end(),
+                         [b](size_t x, size_t y) { return b->cubes[x].n > b->cubes[y].n; });
+        std::vector<int64_t> load(ns, 0);
+        for (size_t i : order) {
+            int s = 0;
+            for (int t = 1; t < ns; ++t)
+                if (load[t] < load[s]) s = t;
+            stream_of[i] = s;
+            load[s] += b->cubes[i].n;
+        }
+    } else {
+        for (size_t i = 0; i < nc; ++i) stream_of[i] = static_cast<int>(i % ns);
+    }
     std::vector<hipStream_t> lane_stream(ns);
     for (int s = 0; s < ns && e == hipSuccess; ++s) {
         lane_stream[s] = b->cubes[s].ctx->stream;
         e = hipStreamWaitEvent(lane_stream[s], ev[ns], 0);
     }
-    for (size_t i = 0; i < b->cubes.size() && e == hipSuccess && rc == ASM_OK; ++i) {
+    for (size_t k = 0; k < nc && e == hipSuccess && rc == ASM_OK; ++k) {
+        const size_t i = order[k];
         asm_batch::Cube& c = b->cubes[i];
         hipStream_t own = c.ctx->stream;
-        c.ctx->stream = lane_stream[i % ns];
+        c.ctx->stream = lane_stream[stream_of[i]];
         c.ctx->capturing = true;
         rc = run_cube(c, b->totals.p + b->total_off[i]);
         c.ctx->capturing = false;

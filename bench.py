@@ -156,14 +156,23 @@ def batched_leg(steps: int, n_streams: int = 8):
     log(f"[bench] batched: {len(cubes)} shares, {n_local} local / {n_buf} buffered particles "
         f"({time.time() - t:.1f} s to generate)")
     leaf, tiles = b.autotune((16, 24, 32))
-    b.capture(n_streams)
-    b.launch()
-    b.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+
+    def timed_graph():
+        b.capture(n_streams)
         b.launch()
-    b.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.launch()
+        b.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    # Shares assigned to the streams largest first, each to the stream with
+    # the fewest particles so far, against the default round-robin order.
+    b.set_option(A.OPT_BATCH_ORDER, 1)
+    ms_balanced = timed_graph()
+    b.set_option(A.OPT_BATCH_ORDER, 0)
+    ms = timed_graph()
     graph_tot = [[b.get_mesh(i, nc)[1] for nc in ncs] for i in range(len(cubes))]
     t0 = time.perf_counter()
     b.run_sequential()
@@ -180,9 +189,16 @@ def batched_leg(steps: int, n_streams: int = 8):
             ratios.append((tot[0] + tot[1]) / nl)
             g = graph_tot[i][j]
             diff = max(diff, abs((g[0] + g[1]) - (tot[0] + tot[1])) / nl)
+    n_cells = sum(nc ** 3 for nc in ncs) * len(cubes)
     b.close()
-    log(f"[bench] batched: {ms:.2f} ms per graph launch, {seq_ms:.2f} ms one share after another")
+    log(f"[bench] batched: {ms:.2f} ms per graph launch ({ms_balanced:.2f} ms largest first), "
+        f"{seq_ms:.2f} ms one share after another")
+    # The path's compulsory traffic (80 B per buffered particle, 24 B per
+    # cell) over the graph launch, against the HBM peak.
+    b_alg = 80 * n_buf + 24 * n_cells
     return {"value": round(n_local / (ms * 1e-3), 1), "unit": "particles/s", "ms_per_step": round(ms, 3),
+            "largest_first_ms": round(ms_balanced, 3),
+            "path_roofline_frac": round(b_alg / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 6),
             "sequential_ms": round(seq_ms, 3), "gain_vs_sequential": round(seq_ms / ms, 4),
             "shares": len(cubes), "n_local": n_local, "n_buf": n_buf, "ncs": ncs, "streams": n_streams,
             "autotuned": {"leaf_size": leaf, "tiles": dict(zip(map(str, ncs), tiles))},

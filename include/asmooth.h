@@ -83,12 +83,15 @@ enum {
                                       the separable cubic B-spline of half-width h, integrated
                                       exactly over each cell, on the mesh.  Setting it requires
                                       a new asm_neighbor_search. */
-    ASM_OPT_KNN_BOUND = 10      /* first walk of ASM_OPT_KNN_METHOD 0 (the 33rd distance of
+    ASM_OPT_KNN_BOUND = 10,     /* first walk of ASM_OPT_KNN_METHOD 0 (the 33rd distance of
                                       every particle): 0 (default) selection, one query at a
                                       time over its wavefront with a bitonic sort of the
                                       candidates within a seed-range bound; 1 the packet walk
                                       (64 queries share every candidate, med3 insertion).
                                       Same results.  With a seed model the packet walk runs. */
+    ASM_OPT_BATCH_ORDER = 11    /* asm_batch_set_option only: 0 (default) share i on stream
+                                      i % n_streams in the order added; 1 largest share first,
+                                      each onto the stream with the fewest particles so far */
 };
 
 const char* asm_version(void);

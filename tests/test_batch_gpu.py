@@ -21,7 +21,10 @@ def _shares(k=5):
     return out, bf
 
 
-def test_batch_graph_matches_single_path(ctx):
+@pytest.mark.parametrize("order", [0, 1])
+def test_batch_graph_matches_single_path(ctx, order):
+    """Both stream assignments (ASM_OPT_BATCH_ORDER 0: round robin in the
+    order added; 1: largest share first onto the least-loaded stream)."""
     shares, bf = _shares()
     L = 32.0
     ncs = [4, 8]
@@ -30,6 +33,7 @@ def test_batch_graph_matches_single_path(ctx):
     try:
         for p, nl in shares:
             b.add(p, nl, L, -bf * L, (1 + bf) * L, ncs, -1)
+        b.set_option(A.OPT_BATCH_ORDER, order)
         b.capture(n_streams=2)
         for replay in range(3):
             b.launch()
