@@ -721,81 +721,6 @@ __device__ __forceinline__ uint32_t wave_sort64(uint32_t v, int lane)
     return v;
 }
 
-// Lane l of a row of 16 reads lane l ^ M of the same row: DPP quad
-// permutations (xor 1, 2, 3), half-row mirror (xor 7), row mirror (xor 15),
-// and their compositions for xor 4 (7 then 3) and xor 8 (15 then 7).
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_mov(uint32_t v)
-{
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
-}
-template <int M>
-__device__ __forceinline__ uint32_t row_xor(uint32_t v)
-{
-    if constexpr (M == 1) return dpp_mov<0xB1>(v);
-    else if constexpr (M == 2) return dpp_mov<0x4E>(v);
-    else if constexpr (M == 3) return dpp_mov<0x1B>(v);
-    else if constexpr (M == 7) return dpp_mov<0x141>(v);
-    else if constexpr (M == 15) return dpp_mov<0x140>(v);
-    else if constexpr (M == 4) return dpp_mov<0x1B>(dpp_mov<0x141>(v));
-    else {
-        static_assert(M == 8, "row xor pattern");
-        return dpp_mov<0x141>(dpp_mov<0x140>(v));
-    }
-}
-__device__ __forceinline__ void cx_reg(uint32_t& a, uint32_t& b)
-{
-    const uint32_t lo = min(a, b);
-    b = max(a, b);
-    a = lo;
-}
-// Compare-exchange of element 4l+v with element 4(l^M)+w, w = 3-v (FLIP,
-// the mirror stage of a merge) or v; the lane with bit LB clear keeps the
-// smaller value.
-template <int M, int LB, bool FLIP>
-__device__ __forceinline__ void cx_row(uint32_t (&x)[4], int l16)
-{
-    uint32_t p[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) p[v] = row_xor<M>(x[FLIP ? 3 - v : v]);
-    const bool lower = (l16 & LB) == 0;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) x[v] = lower ? min(x[v], p[v]) : max(x[v], p[v]);
-}
-// Ascending sort of 64 values held by a row of 16 lanes, element 4l+v in
-// register v of lane l: the mirror bitonic network, its stages between
-// elements of one lane done in registers, the others by DPP within the row.
-// Needs every lane of the row active.
-__device__ __forceinline__ void row_sort64(uint32_t (&x)[4], int l16)
-{
-    cx_reg(x[0], x[1]);
-    cx_reg(x[2], x[3]);
-    const auto in_lane = [&]() {   // half-cleaners of element bits 1 and 0
-        cx_reg(x[0], x[2]);
-        cx_reg(x[1], x[3]);
-        cx_reg(x[0], x[1]);
-        cx_reg(x[2], x[3]);
-    };
-    cx_reg(x[0], x[3]);
-    cx_reg(x[1], x[2]);
-    cx_reg(x[0], x[1]);
-    cx_reg(x[2], x[3]);
-    cx_row<1, 1, true>(x, l16);
-    in_lane();
-    cx_row<3, 2, true>(x, l16);
-    cx_row<1, 1, false>(x, l16);
-    in_lane();
-    cx_row<7, 4, true>(x, l16);
-    cx_row<2, 2, false>(x, l16);
-    cx_row<1, 1, false>(x, l16);
-    in_lane();
-    cx_row<15, 8, true>(x, l16);
-    cx_row<4, 4, false>(x, l16);
-    cx_row<2, 2, false>(x, l16);
-    cx_row<1, 1, false>(x, l16);
-    in_lane();
-}
-
 // Entry r of the list of sorted particle q: one row of kListCap per
 // particle, so that a query's list is one contiguous store.
 __device__ __forceinline__ int64_t list_slot(int64_t q, int r)
