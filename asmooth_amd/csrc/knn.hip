@@ -619,11 +619,12 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
 //  C. Per query: the listed leaves with the query's bit set give their group
 //     boxes, those within the query's bound their particles, and the squared
 //     distances within the bound go to a list in LDS.  The list is cut back
-//     whenever it passes 64 entries: the first 64 are sorted across the
-//     wavefront (a 21-stage bitonic network on the bit patterns, which order
-//     non-negative floats), the smallest 33 are kept and the bound drops to
-//     the 33rd.  At the end the list is sorted once more and tau is its 33rd
-//     entry.  Every particle with d2 <= tau lies within every bound the query
+//     whenever it passes 64 entries: the 33rd smallest of the first 64 is
+//     found across the wavefront (two 32-lane bitonic sorts on the bit
+//     patterns, which order non-negative floats, then a half-cleaner and a
+//     minimum), the entries up to it are kept (33 of them with ties) and the
+//     bound drops to it.  At the end tau is the list's 33rd smallest, found
+//     the same way.  Every particle with d2 <= tau lies within every bound the query
 //     had, so it was listed: tau is the 33rd smallest squared distance over
 //     all particles, ties included, in the reference's FP32 arithmetic.
 //
@@ -691,11 +692,17 @@ __device__ __forceinline__ uint32_t cx_stage(uint32_t v, int lane)
     return (lane & M) ? max(v, p) : min(v, p);
 }
 
-// Ascending sort of one value per lane over the wavefront: a bitonic
-// network whose first stage of each merge of size k compares mirrored lanes
-// (lane ^ (k - 1)), the rest lane ^ j; the smaller value always goes to the
-// lower lane.  Must be called with all 64 lanes active.
-__device__ __forceinline__ uint32_t wave_sort64(uint32_t v, int lane)
+// The 33rd smallest of the 64 lanes' values.  Each half of 32 lanes is
+// sorted ascending by a bitonic network whose first stage of each merge of
+// size k compares mirrored lanes (lane ^ (k - 1)), the rest lane ^ j, the
+// smaller value going to the lower lane.  With A and B the sorted halves,
+// the 32 largest values of the wavefront are max(A[l], B[31 - l]) for
+// l = 0..31 (the upper half of the bitonic half-cleaner on A followed by B
+// reversed), so the 33rd smallest is their minimum.  Lane l and lane 63 - l
+// form the same pair, so every lane holds one of the 32 maxima and a minimum
+// within each half of 32 lanes covers them all.  Must be called with all 64
+// lanes active; returns the value in every lane.
+__device__ __forceinline__ uint32_t wave_33rd(uint32_t v, int lane)
 {
     v = cx_stage<1, 1>(v, lane);
     v = cx_stage<3, 2>(v, lane);
@@ -712,12 +719,13 @@ __device__ __forceinline__ uint32_t wave_sort64(uint32_t v, int lane)
     v = cx_stage<4, 4>(v, lane);
     v = cx_stage<2, 2>(v, lane);
     v = cx_stage<1, 1>(v, lane);
-    v = cx_stage<63, 32>(v, lane);
-    v = cx_stage<16, 16>(v, lane);
-    v = cx_stage<8, 8>(v, lane);
-    v = cx_stage<4, 4>(v, lane);
-    v = cx_stage<2, 2>(v, lane);
-    v = cx_stage<1, 1>(v, lane);
+    v = max(v, xor_lane<63>(v, lane));
+    // minimum over each half: within quads, 8-lane and 16-lane rows, then 32
+    v = min(v, xor_lane<1>(v, lane));
+    v = min(v, xor_lane<2>(v, lane));
+    v = min(v, xor_lane<7>(v, lane));
+    v = min(v, xor_lane<15>(v, lane));
+    v = min(v, xor_lane<31>(v, lane));
     return v;
 }
 
@@ -742,7 +750,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     const Box* __restrict__ groups, uint32_t* __restrict__ list, uint32_t* __restrict__ list_n,
     float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted)
 {
-    static_assert(K <= kWave, "the selection keeps K entries in one wavefront");
+    static_assert(K == 33, "the selection takes the 33rd smallest of 64 (wave_33rd)");
     __shared__ SelectLds lds[kSelWaves];
     const int lane = threadIdx.x % kWave;
     const int wid = wave_in_block();
@@ -989,21 +997,38 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     // within the bound (up to 32 groups per round, four loads per lane),
     // whose squared distances within the bound go to a list.  Whenever the
     // list passes 64 entries the bound drops to the 33rd smallest of its
-    // first 64 (wave_select) and only those at or below it stay, 33 of them
+    // first 64 (wave_33rd) and only those at or below it stay, 33 of them
     // with ties; at the end tau is the 33rd smallest of the list.
     const int gpl = leaf_size / kGroup;      // groups per leaf (leaf_size % kGroup == 0)
     const int lpr = 2 * kWave / gpl;         // leaves per round of group boxes
-    int pr_leaf[2], pr_grp[2];
+    const uint32_t ngroups = static_cast<uint32_t>((ni + kGroup - 1) / kGroup);
+    int pr_leaf[2];
+    uint32_t pr_grp[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int pr = k * kWave + lane;     // (leaf, group) pair of the round
-        pr_leaf[k] = pr / gpl;
-        pr_grp[k] = pr - pr_leaf[k] * gpl;
+        const int l = pr / gpl;
+        pr_grp[k] = static_cast<uint32_t>(pr - l * gpl);
+        // a pair past the round's leaves (gpl not a power of 2) never has one
+        pr_leaf[k] = l < lpr ? l : (1 << 24);
     }
+    const int g_row = lane / kGroup;            // group of a batch of 64 particles ...
+    const uint32_t g_off = static_cast<uint32_t>(lane % kGroup);   // ... and particle in it
     uint32_t* const slist = S.b.c.slist;
     uint32_t* const glist = S.b.c.glist;
     uint32_t* const cand = S.b.c.cand;
     uint32_t* const cpos = S.b.c.cpos;
+    // The union in registers: lane l holds the masks and the first group
+    // of entries l + 64 j (mask zero past nu), so a query's test is a shift.
+    static_assert(kLeafCap == 4 * kWave, "two chunks of 128 union entries");
+    uint64_t um[4];
+    uint32_t ug[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int e = j * kWave + lane;
+        um[j] = e < nu ? S.b.umask[e] : 0ull;
+        ug[j] = e < nu ? S.b.uleaf[e] * static_cast<uint32_t>(gpl) : 0u;
+    }
     uint32_t n_eval = 0, n_acc = 0;
     float tau = 0.0f;
     uint32_t my_n = 0;          // with lists: lane i's list length ...
@@ -1014,68 +1039,97 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
         qq.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.y), i));
         qq.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.z), i));
         float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rA2), i));
-        const uint64_t bit = 1ull << i;
         int cc = 0;
         for (int uc = 0; uc < nu; uc += 2 * kWave) {
+            // slist: the first groups of the union leaves the query needs
             int ns = 0;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int e = uc + h * kWave + lane;
-                const bool has = e < nu && (S.b.umask[e] & bit) != 0ull;
+                const uint64_t m = uc == 0 ? um[h] : um[2 + h];
+                const bool has = ((m >> i) & 1ull) != 0ull;
                 const uint64_t sb = wave_ballot(has);
-                if (has) slist[ns + lane_rank(sb)] = S.b.uleaf[e];
+                if (has) slist[ns + lane_rank(sb)] = uc == 0 ? ug[h] : ug[2 + h];
                 ns += __popcll(sb);
             }
             wave_lds_fence();
             for (int t = 0; t < ns; t += lpr) {
+                // Loads are unconditional (clamped indices); lanes without a
+                // group fail the test through gv.
                 Box gbx[2];
                 bool gv[2];
-                int gidx[2];
+                uint64_t gvm[2];   // ballots of gv, each from compares
+                uint32_t gidx[2];
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     const int r = t + pr_leaf[k];
-                    gv[k] = pr_leaf[k] < lpr && r < ns;
-                    gidx[k] = 0;
-                    if (gv[k]) {
-                        const int p0 = static_cast<int>(slist[r]) * leaf_size + pr_grp[k] * kGroup;
-                        gv[k] = p0 < ni;
-                        gidx[k] = p0 / kGroup;
-                    }
-                    if (gv[k]) gbx[k] = groups[gidx[k]];
+                    const bool rv = r < ns;
+                    const uint32_t g = slist[rv ? r : 0] + pr_grp[k];
+                    const bool g_ok = g < ngroups;
+                    gv[k] = rv && g_ok;
+                    gvm[k] = wave_ballot(rv) & wave_ballot(g_ok);
+                    gidx[k] = g;
+                    gbx[k] = groups[gv[k] ? g : 0u];
                 }
                 int ng = 0;
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
-                    const bool ga = gv[k] && box_d2(gbx[k], qq.x, qq.y, qq.z) <= r2;
-                    const uint64_t gb = wave_ballot(ga);
-                    if (ga) glist[ng + lane_rank(gb)] = static_cast<uint32_t>(gidx[k]);
+                    const float bd = box_d2(gbx[k], qq.x, qq.y, qq.z);
+                    const uint64_t gb = wave_ballot(bd <= r2) & gvm[k];
+                    if (gv[k] && bd <= r2)
+                        glist[__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(gb >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(gb),
+                                                                                  static_cast<uint32_t>(ng)))] = gidx[k];
                     ng += __popcll(gb);
                 }
                 wave_lds_fence();
-                constexpr int kGpb = kWave / kGroup;   // groups per batch of 64 particles
+                // Rounds of up to 32 groups in four batches of 64 particles
+                // (8 groups of kGroup).  A full round gives lane row r the
+                // groups u + 4 r .. u + 4 r + 3 in one 16-byte load, one per
+                // batch; a partial round gives batch k the groups u + 8 k + r,
+                // so that its batches stay full.  Either way batch k holds
+                // groups of row r exactly when 8 k + r < ng - u.
+                constexpr int kGpb = kWave / kGroup;   // groups per batch
+                static_assert(kGpb == 8 && (2 * kWave) % (4 * kGpb) == 0, "rounds of 32 groups");
                 for (int u = 0; u < ng; u += 4 * kGpb) {
+                    const int left = ng - u;
+                    uint32_t g4[4];
+                    if (left >= 4 * kGpb) {
+                        const uint4 v = *reinterpret_cast<const uint4*>(glist + u + 4 * g_row);
+                        g4[0] = v.x;
+                        g4[1] = v.y;
+                        g4[2] = v.z;
+                        g4[3] = v.w;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) g4[k] = glist[u + k * kGpb + g_row];
+                    }
                     float4 cp[4];
                     bool pv[4];
-                    int p_of[4];
+                    uint64_t pvm[4];   // ballots of pv, each from compares
+                    uint32_t p_of[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        const int gi = u + k * kGpb + lane / kGroup;
-                        const int p = gi < ng ? static_cast<int>(glist[gi]) * kGroup + (lane % kGroup) : ni;
-                        pv[k] = p < ni;
+                        const uint32_t p = g4[k] * static_cast<uint32_t>(kGroup) + g_off;
+                        const bool row_ok = k * kGpb + g_row < left;
+                        const bool p_ok = p < static_cast<uint32_t>(ni);
+                        pv[k] = row_ok && p_ok;
+                        pvm[k] = wave_ballot(row_ok) & wave_ballot(p_ok);
                         p_of[k] = p;
-                        cp[k] = pv[k] ? spos[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        cp[k] = spos[pv[k] ? p : static_cast<uint32_t>(w0)];
                     }
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        if (u + k * kGpb >= ng) break;
+                        if (k * kGpb >= left) break;
                         const float d2 = norm2(qq.x, qq.y, qq.z, cp[k].x, cp[k].y, cp[k].z);
                         const bool acc = pv[k] && d2 <= r2;
-                        const uint64_t ab = wave_ballot(acc);
-                        n_eval += static_cast<uint32_t>(__popcll(wave_ballot(pv[k])));
+                        const uint64_t ab = wave_ballot(d2 <= r2) & pvm[k];
+                        if (stats) n_eval += static_cast<uint32_t>(__popcll(pvm[k]));
                         if (acc) {
-                            const int slot = cc + lane_rank(ab);
+                            const int slot = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                static_cast<uint32_t>(ab >> 32),
+                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ab), static_cast<uint32_t>(cc))));
                             cand[slot] = __float_as_uint(d2);
-                            cpos[slot] = static_cast<uint32_t>(p_of[k]);
+                            cpos[slot] = p_of[k];
                         }
                         cc += __popcll(ab);
                         n_acc += static_cast<uint32_t>(__popcll(ab));
@@ -1095,7 +1149,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
                             const uint32_t xe = more ? cand[kWave + lane] : 0xffffffffu;
                             const uint32_t xep = more ? cpos[kWave + lane] : 0u;
                             const uint32_t kb = static_cast<uint32_t>(
-                                __builtin_amdgcn_readlane(static_cast<int>(wave_sort64(x, lane)), K - 1));
+                                __builtin_amdgcn_readfirstlane(static_cast<int>(wave_33rd(x, lane))));
                             r2 = __uint_as_float(kb);
                             const int n_lt = __popcll(wave_ballot(x < kb));
                             const uint64_t eq = wave_ballot(x == kb);
@@ -1124,7 +1178,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
             }
         }
         const uint32_t x = lane < cc ? cand[lane] : 0xffffffffu;
-        const uint32_t tb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wave_sort64(x, lane)), K - 1));
+        const uint32_t tb = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(wave_33rd(x, lane))));
         const float ti = __uint_as_float(tb);
         tau = (lane == i) ? ti : tau;
         if (list) {
