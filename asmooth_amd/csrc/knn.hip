@@ -1513,15 +1513,17 @@ __global__ __launch_bounds__(kCollectThreads) void knn_collect_kernel(
 }
 
 // Density from the selection walk's lists (wavefronts whose flag is clear;
-// the collect walk takes the others).  Each particle q has a row of 16 lanes
-// (four particles per wavefront), lane l taking entries l, l + 16, l + 32 of
-// q's list: every particle c with d2 <= tau_q.  The terms are settled as in
+// the collect walk takes the others).  Each particle q has a row of 8 lanes
+// (eight particles per wavefront), lane l taking entries l, l + 8, ... of
+// q's list: every particle c with d2 <= tau_q.  (Rows of 8 lanes leave fewer
+// lanes idle in a list's last round than rows of 16: 3.2 against 3.9 ms for
+// the 256^3 workload; rows of 4 take 3.7 ms.)  The terms are settled as in
 // collect_leaf, with q as the owner: c's term on q is pulled when d2 < tau_c
 // (summed over the row in FP64); q's term on c is scattered with an FP64
 // atomic when d2 < tau_q and d2 > tau_c (c does not list q), and at
 // d2 = tau_q when c's key is below q's 33rd key (a tie in q's first 32).
 constexpr int kListThreads = 256;
-constexpr int kListRow = 16;
+constexpr int kListRow = 8;
 
 template <int KERN>
 __global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
@@ -1531,8 +1533,8 @@ __global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kListThreads + threadIdx.x;
     const int64_t q = t / kListRow;
-    const int l16 = static_cast<int>(t % kListRow);
-    // A row (16 lanes) shares q: rows past the end and rows of flagged
+    const int lr = static_cast<int>(t % kListRow);
+    // A row shares q: rows past the end and rows of flagged
     // wavefronts leave together, so the row sum below runs on whole rows.
     if (q >= n || wave_flags[q / kWave] != 0u) return;
     const float4 me = spos[q];
@@ -1543,7 +1545,7 @@ __global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
     double pacc = 0.0;
 #pragma unroll
     for (int e0 = 0; e0 < kListCap; e0 += kListRow) {
-        const int e = e0 + l16;
+        const int e = e0 + lr;
         if (e >= cnt) break;
         const uint32_t j = list[list_slot(q, e)];
         const float4 c = spos[j];
@@ -1559,7 +1561,7 @@ __global__ __launch_bounds__(kListThreads) void knn_list_dens_kernel(
     }
 #pragma unroll
     for (int o = kListRow / 2; o > 0; o >>= 1) pacc += __shfl_xor(pacc, o, kListRow);
-    if (l16 == 0) dens_pull[q] = pacc;
+    if (lr == 0) dens_pull[q] = pacc;
 }
 
 // Walk 1 with a learned bound, as a range walk.  Each lane's learned bound
