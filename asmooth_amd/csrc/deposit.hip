@@ -15,6 +15,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace asm_hip {
 namespace {
 
@@ -558,44 +560,57 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
     flush_tile(a, acc, cx0, cy0, cz0, mesh, accumulate);
 }
 
-// Node mesh, one workgroup per tile, with the (particle, cell) terms packed
-// onto lanes.  A wavefront takes 64 list entries at a time (entries `nb`
-// apart, so that they lie far apart in space and their cells rarely
-// coincide): lane l computes entry l's box of tile cells and its per-axis
-// weights (into LDS).  A box of `size` cells is cut into kCellsPerLane
-// parts of q = ceil(size / kCellsPerLane) cells, and slot s of the box holds
-// its cells s, s + q, s + 2q, ...  An exclusive scan of the slot counts lays
-// the boxes' slots end to end, and the wavefront sweeps that sequence 64
-// slots at a time.  Lane l takes slot base + l: the entries whose slots
-// start inside the sweep mark their start in a per-wavefront LDS map (tagged
-// with the sweep, so it never needs clearing), a ballot of the marks gives
-// every lane its entry, and the lane adds the five or one terms of each of
-// its cells with ds_add_f64.  The marks, the ballot and the gathers of the
-// entry's fields are shared by the lane's cells; lanes of one entry still
-// cover consecutive cells (consecutive LDS words) in each part, and each
-// term is the reference's FP32 product ((wx * wy) * wz, coarse_mesh.cpp:93).
+// Node mesh, one workgroup per tile: every (particle, cell) term of the
+// tile's list goes into FP64 accumulators in LDS with ds_add_f64, and the
+// tile is flushed once with plain stores, every cell exactly once.  The time
+// goes to the LDS atomics (five per cell of a mesh particle) and to the
+// vector work that forms the terms, so list-adjacent entries are taken in
+// groups of G whose terms for one cell are summed in registers first.  The
+// tile's list holds the particles of each workgroup of the list pass in
+// Morton order, so G adjacent entries are nearly always neighbouring
+// particles whose boxes of cells almost coincide.
+//
+// A wavefront takes 64 / G groups at a time (groups far apart in the list,
+// so that their cells rarely coincide; entry g of group q on lane G * q + g).
+// Each lane computes its entry's box and writes the entry's per-axis weights
+// in the frame of the group's union box, zero outside its own box.  An
+// exclusive scan lays the union boxes' (x, y) columns end to end, and the
+// wavefront sweeps them 64 at a time: the groups whose columns start inside
+// the sweep mark their start in a per-wavefront LDS map (tagged with the
+// sweep, so it never needs clearing), and a ballot of the marks gives every
+// lane its group.  The lane forms wx * wy of every member for its column once
+// and walks the column's cells: each member's term is the reference's FP32
+// product ((wx * wy) * wz and its products with the fields,
+// coarse_mesh.cpp:93-122), the members' terms are added in FP64 in member
+// order, and each field takes one ds_add_f64 per cell that some member
+// reaches.  Lanes of one group cover consecutive x, so one instruction's
+// atomics hit neighbouring words.
 //
 // Eight wavefronts share a tile's accumulators: the tile's padded FP64 sums
-// (27 KB) plus 6.5 KB of weights and marks per wavefront let two workgroups,
+// (27 KB) plus 6.2 KB of weights and marks per wavefront let two workgroups,
 // 16 wavefronts, share a CU's 160 KB of LDS.
-constexpr int kPackThreads = 512;
-constexpr int kDepWaves = kPackThreads / kWave;
-constexpr int kWStride = 3 * kTileMax + 1;   // odd row stride: entries fall in different banks
-constexpr int kCellsPerLane = 4;             // cells of one box a lane takes per sweep
+constexpr int kTileWaves = 8;
+constexpr int kTileGroup = 4;                // list-adjacent entries summed per atomic
+constexpr int kColWStride = 3 * kTileMax;    // wx[8] wy[8] wz[8] per entry: 96 B, 16-B aligned
 
-template <typename Acc>
-__device__ __forceinline__ void tile_packed_body(const DepositArgs& a, int64_t tile,
-                                                 const uint32_t* __restrict__ tile_start,
-                                                 const uint32_t* __restrict__ tile_cnt,
-                                                 const uint32_t* __restrict__ list, float* __restrict__ mesh,
-                                                 int accumulate)
+template <typename Acc, int kWaves, int G>
+__device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t tile,
+                                                const uint32_t* __restrict__ tile_start,
+                                                const uint32_t* __restrict__ tile_cnt,
+                                                const uint32_t* __restrict__ list, float* __restrict__ mesh,
+                                                int accumulate)
 {
+    static_assert(G == 1 || G == 2 || G == 4, "group size");
+    constexpr int kThreads = kWaves * kWave;
+    constexpr int kGroups = kWave / G;   // groups per batch
     __shared__ Acc acc[6][kPadCells];
-    __shared__ float wts[kDepWaves][kWave * kWStride];
-    __shared__ int smap[kDepWaves][kWave];
+    __shared__ __align__(16) float wts[kWaves][kWave * kColWStride];
+    __shared__ int smap[kWaves][kWave];
     const int tid = threadIdx.x;
     const int lane = tid % kWave;
     const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int g = lane % G;
+    const int qi = lane / G;
     float* const W = wts[wid];
     int* const map = smap[wid];
     const int T = a.tile;
@@ -603,110 +618,204 @@ __device__ __forceinline__ void tile_packed_body(const DepositArgs& a, int64_t t
     const int ty = static_cast<int>((tile / a.ntx) % a.ntx);
     const int tz = static_cast<int>(tile / (static_cast<int64_t>(a.ntx) * a.ntx));
     const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
-    for (int c = tid; c < 6 * kPadCells; c += kPackThreads) (&acc[0][0])[c] = Acc(0);
+    for (int c = tid; c < 6 * kPadCells; c += kThreads) (&acc[0][0])[c] = Acc(0);
     map[lane] = -1;
     __syncthreads();
 
     const float dxc = a.boxsize / static_cast<float>(a.nc);
     const uint32_t begin = tile_start[tile];
     const uint32_t count = min(tile_cnt[tile], begin < a.list_cap ? a.list_cap - begin : 0u);
-    const uint32_t nb = (count + kWave - 1) / kWave;   // batches; batch b holds entries l * nb + b
-    int tag = 0;                                       // sweep counter, tags the map marks
-    for (uint32_t b = wid; b < nb; b += kDepWaves) {
-        const uint32_t e = static_cast<uint32_t>(lane) * nb + b;
-        // Entry setup: box of tile cells [b0, b0 + n) per axis and weights.
-        int size = 0;
-        uint32_t packed = 0;    // b0x | b0y << 4 | b0z << 8 | nx << 12 | ny << 16 | igm << 20 | nz << 21
-        float dens = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
-        if (e < count) {
+    const uint32_t ngrp = (count + G - 1) / G;
+    const uint32_t nb = (ngrp + kGroups - 1) / kGroups;   // batches; batch b holds groups q * nb + b
+    int tag = 0;                                          // sweep counter, tags the map marks
+    for (uint32_t b = wid; b < nb; b += kWaves) {
+        const uint32_t e = (static_cast<uint32_t>(qi) * nb + b) * G + g;
+        const bool valid = e < count;
+        // The entry's box [lo, hi) of tile cells per axis (empty: lo 8, hi 0).
+        int lo[3] = {kTileMax, kTileMax, kTileMax}, hi[3] = {0, 0, 0};
+        float x[3] = {0.0f, 0.0f, 0.0f};
+        float hinv = 0.0f, ds = 0.0f;
+        float4 fe = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // dens, vx, vy, vz
+        int igm = 0;
+        const int c0[3] = {cx0, cy0, cz0};
+        if (valid) {
             const int64_t i = list[begin + e];
             const PartRange r = part_range(a, i);
-            const float hinv = 1.0f / r.h;
-            const float ds = dxc * hinv;
+            hinv = 1.0f / r.h;
+            ds = dxc * hinv;
             const float4 at = a.sattr[i];
-            dens = a.dens[i];
-            vx = at.x;
-            vy = at.y;
-            vz = at.z;
-            const int igm = (a.sign * __float_as_int(at.w) <= 0) ? 1 : 0;
-            const int c0[3] = {cx0, cy0, cz0};
-            int b0[3], n3[3];
+            fe = make_float4(a.dens[i], at.x, at.y, at.z);
+            igm = (a.sign * __float_as_int(at.w) <= 0) ? 1 : 0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const int lo = max(r.lo[d], c0[d]);
-                const int hi = min(r.hi[d], c0[d] + T);
-                b0[d] = lo - c0[d];
-                n3[d] = hi > lo ? hi - lo : 0;
-                for (int t = 0; t < n3[d]; ++t) {
-                    const float s1 = (static_cast<float>(lo + t) * dxc - r.x[d]) * hinv;
-                    W[lane * kWStride + d * kTileMax + t] = cell_integ(a.kern, s1, s1 + ds);
+                x[d] = r.x[d];
+                const int l = max(r.lo[d], c0[d]) - c0[d];
+                const int h = min(r.hi[d], c0[d] + T) - c0[d];
+                if (h > l) {
+                    lo[d] = l;
+                    hi[d] = h;
                 }
             }
-            size = n3[0] * n3[1] * n3[2];
-            packed = static_cast<uint32_t>(b0[0] | (b0[1] << 4) | (b0[2] << 8) | (n3[0] << 12) | (n3[1] << 16) |
-                                           (igm << 20) | (n3[2] << 21));
         }
-        const int slots = (size + kCellsPerLane - 1) / kCellsPerLane;
-        // Exclusive scan of the box slot counts.
-        int incl = slots;
+        // The group's union box and its members' mesh/halo bits.
+        int ulo[3] = {lo[0], lo[1], lo[2]}, uhi[3] = {hi[0], hi[1], hi[2]};
+        int gm = igm << g;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                ulo[d] = min(ulo[d], __shfl_xor(ulo[d], o));
+                uhi[d] = max(uhi[d], __shfl_xor(uhi[d], o));
+            }
+            gm |= __shfl_xor(gm, o);
+        }
+        int un[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) un[d] = uhi[d] > ulo[d] ? uhi[d] - ulo[d] : 0;
+        // Weights in the union frame, zero outside the entry's own box.  A
+        // weight or field that is not finite (h = 0 gives infinite 1/h)
+        // would turn the zeros into NaN; such batches take the walk that
+        // tests every member's own box instead.
+        bool bad = valid && !(isfinite(fe.x) && isfinite(fe.y) && isfinite(fe.z) && isfinite(fe.w));
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            for (int t = 0; t < un[d]; ++t) {
+                const int c = ulo[d] + t;
+                float w = 0.0f;
+                if (c >= lo[d] && c < hi[d]) {
+                    const float s1 = (static_cast<float>(c0[d] + c) * dxc - x[d]) * hinv;
+                    w = cell_integ(a.kern, s1, s1 + ds);
+                    bad = bad || !isfinite(w);
+                }
+                W[lane * kColWStride + d * kTileMax + t] = w;
+            }
+        const bool halo_batch = __any(valid && !igm);
+        const bool clean_batch = !__any(bad);
+        const int obox = lo[0] | (hi[0] << 4) | (lo[1] << 8) | (hi[1] << 12) | (lo[2] << 16) | (hi[2] << 20);
+        const int cols = (g == 0 && un[2] > 0) ? un[0] * un[1] : 0;
+        const uint32_t gpk = static_cast<uint32_t>(ulo[0] | (ulo[1] << 4) | (ulo[2] << 8) | (un[0] << 12) |
+                                                   (un[1] << 16) | (un[2] << 20) | (gm << 24));
+        // Exclusive scan of the column counts (only group leaders have any).
+        int incl = cols;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
             const int v = __shfl_up(incl, o);
             if (lane >= o) incl += v;
         }
-        const int pre = incl - slots;
+        const int pre = incl - cols;
         const int total = __shfl(incl, kWave - 1);
-        int first = -1;  // entry holding the slot just before the sweep
+        int first = -1;  // group holding the column just before the sweep
         for (int base = 0; base < total; base += kWave, ++tag) {
-            if (slots > 0 && pre >= base && pre < base + kWave) map[pre - base] = tag;
+            if (cols > 0 && pre >= base && pre < base + kWave) map[pre - base] = tag;
             wave_lds_fence_dep();
-            const uint64_t sb = __ballot(map[lane] == tag);   // cells where an entry starts
+            const uint64_t sb = __ballot(map[lane] == tag);   // columns where a group starts
             const int pos = base + lane;
-            // Entry of this cell: the entry before the sweep plus the number
-            // of starts at or below this lane.
             const int p = first + __popcll(sb & ((2ull << lane) - 1ull));
             first = __shfl(p, kWave - 1);
-            const int p_pre = __shfl(pre, p);
-            const uint32_t pk = static_cast<uint32_t>(__shfl(static_cast<int>(packed), p));
-            const float p_dens = __shfl(dens, p);
-            const float p_vx = __shfl(vx, p);
-            const float p_vy = __shfl(vy, p);
-            const float p_vz = __shfl(vz, p);
+            const int lead = G * p;
+            const uint32_t pk = static_cast<uint32_t>(__shfl(static_cast<int>(gpk), lead));
+            const int ppre = __shfl(pre, lead);
+            float4 f[G];
+#pragma unroll
+            for (int m = 0; m < G; ++m)
+                f[m] = make_float4(__shfl(fe.x, lead + m), __shfl(fe.y, lead + m), __shfl(fe.z, lead + m),
+                                   __shfl(fe.w, lead + m));
+            int nz = 0, cb = 0, ux = 0, uy = 0, uz = 0;
+            float wxy[G];
+            float wz[G][kTileMax];
+#pragma unroll
+            for (int m = 0; m < G; ++m) {
+                wxy[m] = 0.0f;
+#pragma unroll
+                for (int k = 0; k < kTileMax; ++k) wz[m][k] = 0.0f;
+            }
+            const int gbits = (pk >> 24) & 15;
             if (pos < total) {
                 const int bx = pk & 15, by = (pk >> 4) & 15, bz = (pk >> 8) & 15;
-                const int nx = (pk >> 12) & 15, ny = (pk >> 16) & 15, nz = (pk >> 21) & 15;
-                const bool igm = (pk >> 20) & 1;
-                const int nxy = nx * ny;
-                const int box = nxy * nz;
-                const float* const wp = W + p * kWStride;
-                const auto add_cell = [&](int off) {
-                    const int iz = static_cast<int>((static_cast<float>(off) + 0.5f) *
-                                                    __builtin_amdgcn_rcpf(static_cast<float>(nxy)));
-                    const int rem = off - iz * nxy;
-                    const int iy = static_cast<int>((static_cast<float>(rem) + 0.5f) *
-                                                    __builtin_amdgcn_rcpf(static_cast<float>(nx)));
-                    const int ix = rem - iy * nx;
-                    // coarse_mesh.cpp:93: ker3 = (wx * wy) * wz
-                    float ker3 = wp[ix] * wp[kTileMax + iy];
-                    ker3 = ker3 * wp[2 * kTileMax + iz];
-                    const int c = (bz + iz) * kPadZ + (by + iy) * kPadY + bx + ix;
-                    if (igm) {
-                        atomicAdd(&acc[0][c], static_cast<Acc>(ker3));
-                        atomicAdd(&acc[2][c], static_cast<Acc>(ker3 * p_dens));
-                        atomicAdd(&acc[3][c], static_cast<Acc>(ker3 * p_vx));
-                        atomicAdd(&acc[4][c], static_cast<Acc>(ker3 * p_vy));
-                        atomicAdd(&acc[5][c], static_cast<Acc>(ker3 * p_vz));
-                    } else {
-                        atomicAdd(&acc[1][c], static_cast<Acc>(ker3));
-                    }
-                };
-                const int off = pos - p_pre;
-                const int part = (box + kCellsPerLane - 1) / kCellsPerLane;
-                add_cell(off);
+                const int nx = (pk >> 12) & 15;
+                nz = (pk >> 20) & 15;
+                const int off = pos - ppre;
+                const int iy = static_cast<int>((static_cast<float>(off) + 0.5f) *
+                                                __builtin_amdgcn_rcpf(static_cast<float>(nx)));
+                const int ix = off - iy * nx;
 #pragma unroll
-                for (int k = 1; k < kCellsPerLane; ++k)
-                    if (off + k * part < box) add_cell(off + k * part);
+                for (int m = 0; m < G; ++m) {
+                    const float* const wp = W + (lead + m) * kColWStride;
+                    wxy[m] = wp[ix] * wp[kTileMax + iy];
+                    const float4 z0 = *reinterpret_cast<const float4*>(wp + 2 * kTileMax);
+                    const float4 z1 = *reinterpret_cast<const float4*>(wp + 2 * kTileMax + 4);
+                    wz[m][0] = z0.x; wz[m][1] = z0.y; wz[m][2] = z0.z; wz[m][3] = z0.w;
+                    wz[m][4] = z1.x; wz[m][5] = z1.y; wz[m][6] = z1.z; wz[m][7] = z1.w;
+                }
+                ux = bx + ix;
+                uy = by + iy;
+                uz = bz;
+                cb = bz * kPadZ + uy * kPadY + ux;
             }
+            // The column's cells.  The members' terms (each the reference's
+            // FP32 product) are summed in FP64 in member order, and each
+            // field takes one atomic per cell that some member reaches.
+            const auto walk = [&](auto halo_tag, auto clean_tag) {
+                constexpr bool kHalo = decltype(halo_tag)::value;
+                constexpr bool kClean = decltype(clean_tag)::value;
+                int mbox[G];
+#pragma unroll
+                for (int m = 0; m < G; ++m) mbox[m] = kClean ? 0 : __shfl(obox, lead + m);
+                bool inxy[G];
+#pragma unroll
+                for (int m = 0; m < G; ++m)
+                    inxy[m] = kClean || (ux >= (mbox[m] & 15) && ux < ((mbox[m] >> 4) & 15) &&
+                                         uy >= ((mbox[m] >> 8) & 15) && uy < ((mbox[m] >> 12) & 15));
+#pragma unroll
+                for (int iz = 0; iz < kTileMax; ++iz) {
+                    const bool on = iz < nz;
+                    if (!__any(on)) break;
+                    if (!on) continue;
+                    Acc sn = Acc(0), sh = Acc(0), sc = Acc(0), sx = Acc(0), sy = Acc(0), sz = Acc(0);
+                    bool any_n = false, any_h = false;
+#pragma unroll
+                    for (int m = 0; m < G; ++m) {
+                        bool in = true;
+                        if (!kClean)
+                            in = inxy[m] && uz + iz >= ((mbox[m] >> 16) & 15) && uz + iz < ((mbox[m] >> 20) & 15);
+                        // coarse_mesh.cpp:93: ker3 = (wx * wy) * wz
+                        const float ker3 = wxy[m] * wz[m][iz];
+                        const bool mi = !kHalo || ((gbits >> m) & 1);
+                        const bool use_n = in && mi;
+                        const Acc tn = use_n ? static_cast<Acc>(ker3) : Acc(0);
+                        const Acc tc = use_n ? static_cast<Acc>(ker3 * f[m].x) : Acc(0);
+                        const Acc tx = use_n ? static_cast<Acc>(ker3 * f[m].y) : Acc(0);
+                        const Acc ty = use_n ? static_cast<Acc>(ker3 * f[m].z) : Acc(0);
+                        const Acc tz = use_n ? static_cast<Acc>(ker3 * f[m].w) : Acc(0);
+                        any_n = any_n || (use_n && ker3 != 0.0f);
+                        if (m == 0) {
+                            sn = tn; sc = tc; sx = tx; sy = ty; sz = tz;
+                        } else {
+                            sn += tn; sc += tc; sx += tx; sy += ty; sz += tz;
+                        }
+                        if (kHalo) {
+                            const bool use_h = in && !mi;
+                            sh += use_h ? static_cast<Acc>(ker3) : Acc(0);
+                            any_h = any_h || (use_h && ker3 != 0.0f);
+                        }
+                    }
+                    const int c = cb + iz * kPadZ;
+                    if (any_n) {
+                        atomicAdd(&acc[0][c], sn);
+                        atomicAdd(&acc[2][c], sc);
+                        atomicAdd(&acc[3][c], sx);
+                        atomicAdd(&acc[4][c], sy);
+                        atomicAdd(&acc[5][c], sz);
+                    }
+                    if (kHalo && any_h) atomicAdd(&acc[1][c], sh);
+                }
+            };
+            if (!clean_batch)
+                walk(std::true_type{}, std::false_type{});
+            else if (halo_batch)
+                walk(std::true_type{}, std::true_type{});
+            else
+                walk(std::false_type{}, std::true_type{});
         }
         wave_lds_fence_dep();
     }
@@ -714,27 +823,42 @@ __device__ __forceinline__ void tile_packed_body(const DepositArgs& a, int64_t t
     flush_tile(a, acc, cx0, cy0, cz0, mesh, accumulate);
 }
 
-template <typename Acc>
-__global__ __launch_bounds__(kPackThreads) void deposit_tile_packed_kernel(
+template <typename Acc, int kWaves, int G, bool kAccumulate>
+__global__ __launch_bounds__(kWaves * kWave, 4) void deposit_tile_group_kernel(
     DepositArgs a, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_cnt,
-    const uint32_t* __restrict__ list, float* __restrict__ mesh, int accumulate)
+    const uint32_t* __restrict__ list, float* __restrict__ mesh)
 {
-    tile_packed_body<Acc>(a, blockIdx.x, tile_start, tile_cnt, list, mesh, accumulate);
+    tile_group_body<Acc, kWaves, G>(a, blockIdx.x, tile_start, tile_cnt, list, mesh, kAccumulate);
+}
+
+// The node-mesh tile kernel with the flush mode as a template argument.
+template <typename Acc, int G>
+void launch_group_tiles(const DepositArgs& a, unsigned ntiles, const uint32_t* tile_start, const uint32_t* tile_cnt,
+                        const uint32_t* list, float* mesh, int accumulate, hipStream_t s)
+{
+    if (accumulate)
+        deposit_tile_group_kernel<Acc, kTileWaves, G, true><<<ntiles, kTileWaves * kWave, 0, s>>>(
+            a, tile_start, tile_cnt, list, mesh);
+    else
+        deposit_tile_group_kernel<Acc, kTileWaves, G, false><<<ntiles, kTileWaves * kWave, 0, s>>>(
+            a, tile_start, tile_cnt, list, mesh);
 }
 
 // The tiles of every mesh of an nc list in one launch (FP64 sums): workgroup
 // w runs tile w - J.tile_base[j] of the mesh j whose range holds w.  The
 // meshes come in the caller's order; with coarse meshes first, their few
 // long tiles start before the many short ones of the fine meshes.
-__global__ __launch_bounds__(kPackThreads) void multi_tile_kernel(MultiDeposit J, const uint32_t* __restrict__ tile_start,
-                                                                  const uint32_t* __restrict__ tile_cnt,
-                                                                  const uint32_t* __restrict__ list)
+template <int kWaves, int G>
+__global__ __launch_bounds__(kWaves * kWave, 4) void multi_group_kernel(MultiDeposit J,
+                                                                     const uint32_t* __restrict__ tile_start,
+                                                                     const uint32_t* __restrict__ tile_cnt,
+                                                                     const uint32_t* __restrict__ list)
 {
     const uint32_t w = blockIdx.x;
     int j = 0;
     while (j + 1 < J.n && w >= J.tile_base[j + 1]) ++j;
     const uint32_t off = J.tile_base[j];
-    tile_packed_body<double>(J.a[j], w - off, tile_start + off, tile_cnt + off, list, J.mesh[j], 0);
+    tile_group_body<double, kWaves, G>(J.a[j], w - off, tile_start + off, tile_cnt + off, list, J.mesh[j], 0);
 }
 
 // Precision study: the tile's sums in FP32 with Kahan compensation.  One
@@ -1011,7 +1135,7 @@ hipError_t launch_multi_tiles(const MultiDeposit& J, const uint32_t* tile_start,
         if (J.a[j].tile < 1 || J.a[j].tile > kTileMax || J.a[j].periodic) return hipErrorInvalidValue;
     const uint32_t ntiles = J.tile_base[J.n];
     if (ntiles == 0) return hipSuccess;
-    multi_tile_kernel<<<ntiles, kPackThreads, 0, s>>>(J, tile_start, tile_cnt, list);
+    multi_group_kernel<kTileWaves, kTileGroup><<<ntiles, kTileWaves * kWave, 0, s>>>(J, tile_start, tile_cnt, list);
     return hipGetLastError();
 }
 
@@ -1026,12 +1150,23 @@ hipError_t launch_deposit_tiles(const DepositArgs& a, const uint32_t* tile_start
     else if (a.accum == kAccumKahan32)
         deposit_tile_kahan_kernel<<<static_cast<unsigned>(ntiles), kKahanThreads, 0, s>>>(a, tile_start, tile_cnt,
                                                                                           list, mesh, accumulate);
-    else if (a.accum == kAccumAtomic32)
-        deposit_tile_packed_kernel<float><<<static_cast<unsigned>(ntiles), kPackThreads, 0, s>>>(
-            a, tile_start, tile_cnt, list, mesh, accumulate);
-    else
-        deposit_tile_packed_kernel<double><<<static_cast<unsigned>(ntiles), kPackThreads, 0, s>>>(
-            a, tile_start, tile_cnt, list, mesh, accumulate);
+    else if (a.accum == kAccumAtomic32)   // one atomic per term, as the reference's FP32 sums
+        launch_group_tiles<float, 1>(a, static_cast<unsigned>(ntiles), tile_start, tile_cnt, list, mesh, accumulate, s);
+    else if (accumulate)
+        launch_group_tiles<double, kTileGroup>(a, static_cast<unsigned>(ntiles), tile_start, tile_cnt, list, mesh,
+                                               accumulate, s);
+    else {
+        // The one-mesh case of the nc-list launch (its register allocation
+        // spills nothing).
+        MultiDeposit J{};
+        J.a[0] = a;
+        J.mesh[0] = mesh;
+        J.tile_base[0] = 0;
+        J.tile_base[1] = static_cast<uint32_t>(ntiles);
+        J.n = 1;
+        multi_group_kernel<kTileWaves, kTileGroup><<<static_cast<unsigned>(ntiles), kTileWaves * kWave, 0, s>>>(
+            J, tile_start, tile_cnt, list);
+    }
     return hipGetLastError();
 }
 

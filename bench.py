@@ -338,7 +338,7 @@ def main(argv=None) -> int:
                         "density from the selection's lists (stage timer: with the collect walk for the "
                         "wavefronts not listed); read x, write h and dens: 20 B/particle x N_buf"),
         "deposit_tiles": (sum(BYTES_DEPOSIT * nbuf + BYTES_CELL * nc ** 3 for nc in ncs) / dep_launches,
-                          "multi_tile_kernel",
+                          "multi_group_kernel",
                           "tile deposition of every mesh of the nc list in one launch; "
                           "36 B/particle x N_buf + 24 B/cell x nc^3, summed over the list"),
         "sort": (BYTES_TREE * nbuf, "rs_scatter", "Morton radix sort; 24 B/particle"),

@@ -225,6 +225,62 @@ def test_bound_walks_on_ties_match_oracle(ctx, name, leaf, bound):
     assert np.array_equal(nbr, onbr)
 
 
+def _check_mesh_nonfinite(gm, om, nc, vel_scale):
+    """_check_mesh where the oracle has non-finite cells: NaN and infinities
+    (with their signs) in exactly the same cells of every field, the finite
+    cells to the usual bars."""
+    g = mesh_fields(gm, nc)
+    o = mesh_fields(om, nc)
+    for f in ("n", "halo", "c", "v"):
+        assert np.array_equal(np.isnan(g[f]), np.isnan(o[f])), f
+        assert np.array_equal(np.isposinf(g[f]), np.isposinf(o[f])), f
+        assert np.array_equal(np.isneginf(g[f]), np.isneginf(o[f])), f
+    for f in ("n", "halo", "c"):
+        ok = np.isfinite(o[f])
+        assert rel_err(g[f][ok], o[f][ok], floor=1e-30) <= MESH_RTOL, f
+    okv = np.isfinite(o["v"])
+    scale = np.repeat(o["n"], 3) * vel_scale + 1e-30
+    assert np.max(np.abs(g["v"][okv] - o["v"][okv]) / scale[okv], initial=0.0) <= MESH_RTOL
+
+
+@pytest.mark.parametrize("name,inf_vel", [("quantized30", False), ("lattice7", False), ("quantized30", True)])
+def test_deposition_on_tie_heavy_sets(ctx, name, inf_vel):
+    """Deposition of the tie-heavy sets with a quarter of the particles in
+    halos (the halo field): one mesh at a time and every mesh in one pass,
+    against the oracle.  With inf_vel one particle moves at an infinite
+    speed: as in the reference (coarse_mesh.cpp:95-116), the momentum cells
+    of its box hold infinities (NaN where its weight is zero), and no other
+    cell may."""
+    pos = _tie_heavy_sets()[name]
+    rng = np.random.default_rng(11)
+    vel = rng.normal(0, 3, pos.shape).astype(np.float32)
+    if inf_vel:
+        vel[7] = (np.inf, -np.inf, 1.0)
+    parts = A.particles_from_xv(np.concatenate([pos, vel], 1))
+    parts["igrp"] = np.where(rng.random(len(pos)) < 0.25, -1, 0).astype(np.int32)
+    if inf_vel:
+        parts["igrp"][7] = 0
+    lo, hi = float(pos.min()) - 0.5, float(pos.max()) + 0.5
+    L = hi - lo
+    onbr, _, ork = O.knn(pos)
+    od32 = O.density(pos, onbr, ork).astype(np.float32)
+    ncs = (5, 8, 3)
+    om = {nc: O.assign_on_mesh(pos, vel, od32, ork, parts["igrp"].copy(), L, nc, -1) for nc in ncs}
+    if inf_vel:
+        assert not np.isfinite(om[8][3 * 8 ** 3:]).all()
+    ctx.set_option(A.OPT_KEEP_NEIGHBORS, 0)
+    ctx.set_particles(parts, len(pos), L)
+    ctx.construct(lo, hi)
+    ctx.neighbor_search(32)
+    vmax = float(np.max(np.abs(vel[np.isfinite(vel)])))
+    for nc in ncs:
+        ctx.assign_on_mesh(nc, -1)
+        _check_mesh_nonfinite(ctx.get_mesh(nc), om[nc], nc, vmax)
+    ctx.assign_on_meshes(list(ncs), -1)
+    for nc in ncs:
+        _check_mesh_nonfinite(ctx.get_mesh(nc), om[nc], nc, vmax)
+
+
 @pytest.mark.parametrize("kind,n_side,seed", [("zeldovich", 20, 4), ("blobs", 20, 6), ("uniform", 16, 8)])
 def test_bound_walks_agree(ctx, kind, n_side, seed):
     xv, L, bf = small_set(kind, n_side, seed)
