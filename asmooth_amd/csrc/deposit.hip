@@ -628,9 +628,16 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
     const uint32_t ngrp = (count + G - 1) / G;
     const uint32_t nb = (ngrp + kGroups - 1) / kGroups;   // batches; batch b holds groups q * nb + b
     int tag = 0;                                          // sweep counter, tags the map marks
+    // The list entry of the wavefront's next batch is read one batch ahead,
+    // so that only the particle loads wait at the top of each batch.
+    const auto entry_of = [&](uint32_t bb) { return (static_cast<uint32_t>(qi) * nb + bb) * G + g; };
+    uint32_t next_idx = 0;
+    if (wid < nb && entry_of(wid) < count) next_idx = list[begin + entry_of(wid)];
     for (uint32_t b = wid; b < nb; b += kWaves) {
-        const uint32_t e = (static_cast<uint32_t>(qi) * nb + b) * G + g;
+        const uint32_t e = entry_of(b);
         const bool valid = e < count;
+        const uint32_t cur_idx = next_idx;
+        if (b + kWaves < nb && entry_of(b + kWaves) < count) next_idx = list[begin + entry_of(b + kWaves)];
         // The entry's box [lo, hi) of tile cells per axis (empty: lo 8, hi 0).
         int lo[3] = {kTileMax, kTileMax, kTileMax}, hi[3] = {0, 0, 0};
         float x[3] = {0.0f, 0.0f, 0.0f};
@@ -639,7 +646,7 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
         int igm = 0;
         const int c0[3] = {cx0, cy0, cz0};
         if (valid) {
-            const int64_t i = list[begin + e];
+            const int64_t i = cur_idx;
             const PartRange r = part_range(a, i);
             hinv = 1.0f / r.h;
             ds = dxc * hinv;
