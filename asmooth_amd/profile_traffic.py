@@ -17,6 +17,7 @@ per-wavefront instruction table.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import sys
 
 import asmooth_amd as A
@@ -125,15 +126,31 @@ def summarize_busy(counters_csv: str, stats_csv: str, command: str, json_out: st
     import json
 
     vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    grids = collections.defaultdict(set)
     for r in csv.DictReader(open(counters_csv)):
         vals[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        grids[_short(r["Kernel_Name"])].add(int(r["Grid_Size"]))
     avg_ms = {_short(r["Name"]): float(r["AverageNs"]) / 1e6 for r in csv.DictReader(open(stats_csv))}
+    # A kernel launched with several grids (the tile kernel serves one mesh
+    # and the whole nc list): the mean over the trace's launches with the
+    # counter run's grids, from the kernel-trace file beside the stats file.
+    trace_csv = stats_csv.replace("_kernel_stats.csv", "_kernel_trace.csv")
+    if trace_csv != stats_csv and os.path.exists(trace_csv):
+        durs = collections.defaultdict(list)
+        for r in csv.DictReader(open(trace_csv)):
+            durs[(_short(r["Kernel_Name"]), int(r["Grid_Size_X"]))].append(
+                (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+        for k, gs in grids.items():
+            d = [x for g in gs for x in durs.get((k, g), [])]
+            if d:
+                avg_ms[k] = sum(d) / len(d)
     mean = lambda xs: sum(xs) / len(xs) if xs else 0.0
     out = ["Command (from the repository root, TMPDIR=/tmp):", "", f"    {command}", "",
            "Percentages are rocprofv3's derived OccupancyPercent, VALUUtilization, SALUBusy and "
            "LDSBankConflict, averaged over a kernel's dispatches.  FP32 GFLOP/launch = 64 x "
            "SQ_INSTS_VALU_FLOPS_FP32; TFLOP/s uses the kernel's mean duration in the kernel-trace run "
-           f"(profiles/traffic_zeldovich256.md); peak FP32 vector rate {FP32_PEAK_TFLOPS} TFLOP/s.", "",
+           "(profiles/traffic_zeldovich256.md) over its launches with the same grid as the counter run; "
+           f"peak FP32 vector rate {FP32_PEAK_TFLOPS} TFLOP/s.", "",
            "| kernel | occupancy | VALU lanes active | SALU busy | LDS bank conflict | FP32 GFLOP/launch | "
            "TFLOP/s | of FP32 peak |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
     js = {"fp32_peak_tflops": FP32_PEAK_TFLOPS, "kernels": {}}
