@@ -94,6 +94,21 @@ struct asm_context {
     // entries than its buffer holds (a replay on denser inputs than the
     // eager run that sized it); null outside batches.
     uint32_t* overflow_flag = nullptr;
+    // Outside batches, a deposition whose tile lists fit the list buffer of
+    // an earlier call reads nothing back: it compares its entry total with
+    // the buffer's capacity on the device (one flag word per call) and is
+    // recorded here.  The next call that reads a mesh or changes what a
+    // deposition reads (settle_depositions) reads the flags and redoes, with
+    // the list sized from its total, every deposition that did not fit.
+    struct PendingDeposit {
+        std::vector<int> ncs;   // the meshes of the call
+        int sign = 0;
+        int periodic = 0;
+        int coord[3] = {0, 0, 0};
+        bool multi = false;     // asm_assign_on_meshes
+    };
+    std::vector<PendingDeposit> dep_pending;
+    DevBuf<uint32_t> dep_flags;
     DevBuf<ParticleRec> input;   // caller order
     // sort / tree
     DevBuf<uint32_t> keys, vals, keys_tmp, vals_tmp, hist;
@@ -269,6 +284,16 @@ int load_common(asm_context* ctx, int64_t n, int64_t np_local, float boxsize)
     return ASM_OK;
 }
 
+int settle_depositions(asm_context* ctx);
+
+// Redo the pending depositions that did not fit before the context is read
+// or changed (a no-op when none is pending).
+#define SETTLE(ctx)                                  \
+    do {                                             \
+        const int settle_rc_ = settle_depositions(ctx); \
+        if (settle_rc_) return settle_rc_;           \
+    } while (0)
+
 }  // namespace
 
 extern "C" {
@@ -316,6 +341,7 @@ const char* asm_last_error(const asm_context* ctx)
 int asm_set_option(asm_context* ctx, int option, int value)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     switch (option) {
     case ASM_OPT_KEEP_NEIGHBORS: ctx->keep_neighbors = value ? 1 : 0; return ASM_OK;
     case ASM_OPT_PROFILE: ctx->profile = value ? 1 : 0; return ASM_OK;
@@ -360,6 +386,7 @@ int asm_set_option(asm_context* ctx, int option, int value)
 int asm_synchronize(asm_context* ctx)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     HIPC(hipSetDevice(ctx->device));
     HIPC(hipStreamSynchronize(ctx->stream));
     HIPC(resolve_timers(ctx));
@@ -370,6 +397,7 @@ int asm_set_particles(asm_context* ctx, const asm_particle* particles, int64_t n
                       float boxsize)
 {
     if (!ctx || !particles) return fail(ctx, ASM_ERR_ARG, "null argument");
+    SETTLE(ctx);
     int rc = load_common(ctx, n, np_local, boxsize);
     if (rc) return rc;
     HIPC(hipSetDevice(ctx->device));
@@ -384,6 +412,7 @@ int asm_set_particles_device(asm_context* ctx, const asm_particle* d_particles, 
                              float boxsize)
 {
     if (!ctx || !d_particles) return fail(ctx, ASM_ERR_ARG, "null argument");
+    SETTLE(ctx);
     int rc = load_common(ctx, n, np_local, boxsize);
     if (rc) return rc;
     HIPC(hipSetDevice(ctx->device));
@@ -396,6 +425,7 @@ int asm_set_particles_device(asm_context* ctx, const asm_particle* d_particles, 
 int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (!ctx->have_particles) return fail(ctx, ASM_ERR_STATE, "asm_tree_construct before asm_set_particles");
     if (!(right > left)) return fail(ctx, ASM_ERR_ARG, "tree cube [%g, %g] is empty", left, right);
     HIPC(hipSetDevice(ctx->device));
@@ -440,6 +470,7 @@ int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode
 int asm_neighbor_search(asm_context* ctx, int k)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (k != kSlots - 1) return fail(ctx, ASM_ERR_ARG, "k must be %d (33 slots incl. self), got %d", kSlots - 1, k);
     if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "asm_neighbor_search before asm_tree_construct");
     HIPC(hipSetDevice(ctx->device));
@@ -524,6 +555,7 @@ int asm_mark_halo_particles(asm_context* ctx, const asm_halo* halos, int64_t n_h
                             int64_t* np_halo, double* m_halo)
 {
     if (!ctx || (n_halo > 0 && !halos)) return fail(ctx, ASM_ERR_ARG, "null argument");
+    SETTLE(ctx);
     if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "asm_mark_halo_particles before asm_tree_construct");
     HIPC(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
@@ -555,7 +587,48 @@ int asm_mark_halo_particles(asm_context* ctx, const asm_halo* halos, int64_t n_h
 
 namespace {
 
-int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coord)
+constexpr size_t kMaxPendingDeposits = 16;
+
+// Whether a deposition of meshes `ncs` may skip the read-back of its list
+// size: a caller's own call (defer_ok: not the autotuner's, a batch's or a
+// redo), outside batches and capture, with a list buffer from an earlier
+// call.  A pending deposition of one of the same meshes is settled first,
+// so that a later redo never overwrites a newer mesh.
+int may_defer(asm_context* ctx, const std::vector<int>& ncs, bool defer_ok, bool* defer)
+{
+    *defer = false;
+    if (!defer_ok || ctx->capturing || ctx->overflow_flag || ctx->list.cap == 0) return ASM_OK;
+    bool overlap = false;
+    for (const auto& p : ctx->dep_pending)
+        for (int nc : p.ncs)
+            if (std::find(ncs.begin(), ncs.end(), nc) != ncs.end()) overlap = true;
+    if (overlap || ctx->dep_pending.size() >= kMaxPendingDeposits) {
+        const int rc = settle_depositions(ctx);
+        if (rc) return rc;
+    }
+    HIPC(ctx->dep_flags.ensure(kMaxPendingDeposits));
+    *defer = true;
+    return ASM_OK;
+}
+
+// Size the list buffer from the counted total (read back), with an eighth
+// more for the next call on slightly denser inputs.
+int size_list(asm_context* ctx, const char* what, int nc, int tile)
+{
+    uint64_t total = 0;
+    HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    // List offsets are 32-bit: more (particle, tile) entries than that
+    // would wrap them.
+    if (total > 0xffffffffull)
+        return fail(ctx, ASM_ERR_CAPACITY, "%s (nc %d, tile %d) needs %llu tile-list entries, more than 2^32 - 1; use larger tiles",
+                    what, nc, tile, static_cast<unsigned long long>(total));
+    const uint64_t want = total + total / 8 + 1;
+    HIPC(ctx->list.ensure(want < 0xffffffffull ? want : 0xffffffffull));
+    return ASM_OK;
+}
+
+int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coord, bool defer_ok = false)
 {
     if (nc <= 0 || nc > 2048) return fail(ctx, ASM_ERR_ARG, "nc %d outside 1..2048", nc);
     if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "mesh assignment before asm_neighbor_search");
@@ -592,7 +665,11 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
     HIPC(ctx->tile_cur.ensure(ntiles));
     HIPC(ctx->scan_total.ensure(1));
     HIPC(ctx->scan_tmp.ensure(static_cast<size_t>(scan_scratch_entries(ntiles))));
-    uint64_t total = 0;
+    bool defer = false;
+    {
+        const int rc = may_defer(ctx, std::vector<int>{nc}, defer_ok, &defer);
+        if (rc) return rc;
+    }
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cnt.p, ntiles, s));
@@ -600,18 +677,23 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
         HIPC(hipMemcpyAsync(ctx->tile_start.p, ctx->tile_cnt.p, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToDevice, s));
         HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, ctx->scan_tmp.p, s));
     }
-    if (!ctx->capturing) {
-        HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
-        // List offsets are 32-bit: more (particle, tile) entries than that
-        // would wrap them.
-        if (total > 0xffffffffull)
-            return fail(ctx, ASM_ERR_CAPACITY, "nc %d with tile %d needs %llu list entries, more than 2^32 - 1; use a larger tile",
-                        nc, a.tile, static_cast<unsigned long long>(total));
-        HIPC(ctx->list.ensure(total > 0 ? total : 1));
+    if (!ctx->capturing && !defer) {
+        const int rc = size_list(ctx, "the mesh", nc, a.tile);
+        if (rc) return rc;
     }
     a.list_cap = static_cast<uint32_t>(ctx->list.cap < 0xffffffffull ? ctx->list.cap : 0xffffffffull);
     if (ctx->overflow_flag) HIPC(launch_flag_overflow(ctx->scan_total.p, a.list_cap, ctx->overflow_flag, s));
+    if (defer) {
+        uint32_t* const flag = ctx->dep_flags.p + ctx->dep_pending.size();
+        HIPC(launch_fill_zero_u32(flag, 1, s));
+        HIPC(launch_flag_overflow(ctx->scan_total.p, a.list_cap, flag, s));
+        asm_context::PendingDeposit pd;
+        pd.ncs = {nc};
+        pd.sign = sign;
+        pd.periodic = periodic;
+        for (int d = 0; d < 3; ++d) pd.coord[d] = a.coord[d];
+        ctx->dep_pending.push_back(pd);
+    }
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cur.p, ntiles, s));
@@ -629,7 +711,7 @@ int deposit_one(asm_context* ctx, int nc, int sign, int periodic, const int* coo
 // concatenated tile counters, one read-back of the list total (none while a
 // batch records) and one launch over all meshes' tiles, coarse meshes first.
 // Other accumulation modes take the per-mesh path.
-int deposit_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
+int deposit_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign, bool defer_ok = false)
 {
     std::vector<int> list_nc;
     for (int i = 0; i < n_nc; ++i) {
@@ -638,7 +720,7 @@ int deposit_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
     }
     if (ctx->deposit_accum != kAccumFp64 || static_cast<int>(list_nc.size()) > kMaxMeshes) {
         for (int nc : list_nc) {
-            const int rc = deposit_one(ctx, nc, sign, 0, nullptr);
+            const int rc = deposit_one(ctx, nc, sign, 0, nullptr, defer_ok);
             if (rc) return rc;
         }
         return ASM_OK;
@@ -682,6 +764,11 @@ int deposit_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
     HIPC(ctx->tile_cur.ensure(ntiles));
     HIPC(ctx->scan_total.ensure(1));
     HIPC(ctx->scan_tmp.ensure(static_cast<size_t>(scan_scratch_entries(ntiles))));
+    bool defer = false;
+    {
+        const int rc = may_defer(ctx, list_nc, defer_ok, &defer);
+        if (rc) return rc;
+    }
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cnt.p, ntiles, s));
@@ -689,19 +776,23 @@ int deposit_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
         HIPC(hipMemcpyAsync(ctx->tile_start.p, ctx->tile_cnt.p, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToDevice, s));
         HIPC(launch_scan_exclusive(ctx->tile_start.p, ntiles, ctx->scan_total.p, ctx->scan_tmp.p, s));
     }
-    if (!ctx->capturing) {
-        uint64_t total = 0;
-        HIPC(hipMemcpyAsync(&total, ctx->scan_total.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
-        if (total > 0xffffffffull)
-            return fail(ctx, ASM_ERR_CAPACITY,
-                        "the nc list needs %llu tile-list entries, more than 2^32 - 1; use larger tiles",
-                        static_cast<unsigned long long>(total));
-        HIPC(ctx->list.ensure(total > 0 ? total : 1));
+    if (!ctx->capturing && !defer) {
+        const int rc = size_list(ctx, "the nc list", list_nc.back(), J.a[J.n - 1].tile);
+        if (rc) return rc;
     }
     const uint32_t cap = static_cast<uint32_t>(ctx->list.cap < 0xffffffffull ? ctx->list.cap : 0xffffffffull);
     for (int j = 0; j < J.n; ++j) J.a[j].list_cap = cap;
     if (ctx->overflow_flag) HIPC(launch_flag_overflow(ctx->scan_total.p, cap, ctx->overflow_flag, s));
+    if (defer) {
+        uint32_t* const flag = ctx->dep_flags.p + ctx->dep_pending.size();
+        HIPC(launch_fill_zero_u32(flag, 1, s));
+        HIPC(launch_flag_overflow(ctx->scan_total.p, cap, flag, s));
+        asm_context::PendingDeposit pd;
+        pd.ncs = list_nc;
+        pd.sign = sign;
+        pd.multi = true;
+        ctx->dep_pending.push_back(pd);
+    }
     {
         StageTimer t(ctx, T_DEPOSIT_LIST);
         HIPC(launch_fill_zero_u32(ctx->tile_cur.p, ntiles, s));
@@ -714,6 +805,28 @@ int deposit_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
     return ASM_OK;
 }
 
+int settle_depositions(asm_context* ctx)
+{
+    if (ctx->dep_pending.empty()) return ASM_OK;
+    std::vector<asm_context::PendingDeposit> pend;
+    pend.swap(ctx->dep_pending);
+    std::vector<uint32_t> flags(pend.size(), 0u);
+    HIPC(hipSetDevice(ctx->device));
+    HIPC(hipMemcpyAsync(flags.data(), ctx->dep_flags.p, sizeof(uint32_t) * pend.size(), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    // The pending calls cover disjoint meshes (may_defer), so their order
+    // does not matter.
+    for (size_t i = 0; i < pend.size(); ++i) {
+        if (!flags[i]) continue;
+        const auto& p = pend[i];
+        const int rc = p.multi ? deposit_meshes(ctx, p.ncs.data(), static_cast<int>(p.ncs.size()), p.sign)
+                               : deposit_one(ctx, p.ncs[0], p.sign, p.periodic, p.periodic ? p.coord : nullptr);
+        if (rc) return rc;
+    }
+    return ASM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -721,19 +834,19 @@ extern "C" {
 int asm_assign_on_mesh(asm_context* ctx, int nc, int sign)
 {
     if (!ctx) return ASM_ERR_ARG;
-    return deposit_one(ctx, nc, sign, 0, nullptr);
+    return deposit_one(ctx, nc, sign, 0, nullptr, true);
 }
 
 int asm_assign_on_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign)
 {
     if (!ctx || !ncs || n_nc <= 0) return fail(ctx, ASM_ERR_ARG, "empty nc list");
-    return deposit_meshes(ctx, ncs, n_nc, sign);
+    return deposit_meshes(ctx, ncs, n_nc, sign, true);
 }
 
 int asm_assign_on_global_mesh(asm_context* ctx, int nc, int sign, const int coord[3])
 {
     if (!ctx || !coord) return fail(ctx, ASM_ERR_ARG, "null argument");
-    return deposit_one(ctx, nc, sign, 1, coord);
+    return deposit_one(ctx, nc, sign, 1, coord, true);
 }
 
 }  // extern "C"
@@ -767,6 +880,7 @@ int asm_autotune(asm_context* ctx, const int* leaf_sizes, int n_leaf, const int*
                  int* leaf_out, int* tiles_out)
 {
     if (!ctx || (n_leaf > 0 && !leaf_sizes) || (n_nc > 0 && !ncs)) return fail(ctx, ASM_ERR_ARG, "null argument");
+    SETTLE(ctx);
     if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "asm_autotune before asm_neighbor_search");
     HIPC(hipSetDevice(ctx->device));
     const int profile = ctx->profile;
@@ -820,6 +934,7 @@ int asm_autotune(asm_context* ctx, const int* leaf_sizes, int n_leaf, const int*
 int asm_set_seed_model(asm_context* ctx, const asm_mlp* m, float margin)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (!m) {
         ctx->seed_on = false;
         return ASM_OK;
@@ -841,6 +956,7 @@ int asm_set_seed_model(asm_context* ctx, const asm_mlp* m, float margin)
 int asm_get_seed_stats(asm_context* ctx, int64_t out[3])
 {
     if (!ctx || !out) return ASM_ERR_ARG;
+    SETTLE(ctx);
     out[0] = out[1] = 0;
     out[2] = ctx->n;
     if (!ctx->seed_used) return ASM_OK;
@@ -860,6 +976,7 @@ int asm_get_seed_stats(asm_context* ctx, int64_t out[3])
 int asm_seed_samples(asm_context* ctx, const int64_t* idx, int64_t m, float* x, float* target)
 {
     if (!ctx || m < 0 || (m > 0 && (!idx || !x || !target))) return fail(ctx, ASM_ERR_ARG, "null argument");
+    SETTLE(ctx);
     if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "asm_seed_samples before asm_neighbor_search");
     if (m == 0) return ASM_OK;
     for (int64_t b = 0; b < m; ++b)
@@ -890,6 +1007,7 @@ int asm_seed_samples(asm_context* ctx, const int64_t* idx, int64_t m, float* x, 
 int asm_mesh_stats(asm_context* ctx, int nc, double out[4])
 {
     if (!ctx || !out) return ASM_ERR_ARG;
+    SETTLE(ctx);
     auto it = ctx->meshes.find(nc);
     if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
     HIPC(hipSetDevice(ctx->device));
@@ -917,6 +1035,7 @@ int asm_mesh_total(asm_context* ctx, int nc, double totals[2])
 int asm_normalize_mesh(asm_context* ctx, int nc)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     auto it = ctx->meshes.find(nc);
     if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
     HIPC(hipSetDevice(ctx->device));
@@ -930,6 +1049,7 @@ int asm_normalize_mesh(asm_context* ctx, int nc)
 int asm_stitch_mesh(asm_context* ctx, int nc, int nn, const int coord[3], int field, float* dst)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (!coord || !dst) return fail(ctx, ASM_ERR_ARG, "null argument");
     if (nn <= 0 || field < 0 || field > 5) return fail(ctx, ASM_ERR_ARG, "nn %d or field %d out of range", nn, field);
     for (int a = 0; a < 3; ++a)
@@ -947,6 +1067,7 @@ int asm_stitch_mesh(asm_context* ctx, int nc, int nn, const int coord[3], int fi
 int asm_get_mesh(asm_context* ctx, int nc, float* mesh)
 {
     if (!ctx || !mesh) return ASM_ERR_ARG;
+    SETTLE(ctx);
     auto it = ctx->meshes.find(nc);
     if (it == ctx->meshes.end()) return fail(ctx, ASM_ERR_STATE, "no mesh for nc %d", nc);
     HIPC(hipSetDevice(ctx->device));
@@ -958,7 +1079,7 @@ int asm_get_mesh(asm_context* ctx, int nc, float* mesh)
 
 const float* asm_mesh_device_ptr(asm_context* ctx, int nc)
 {
-    if (!ctx) return nullptr;
+    if (!ctx || settle_depositions(ctx)) return nullptr;
     auto it = ctx->meshes.find(nc);
     return it == ctx->meshes.end() ? nullptr : it->second.mesh.p;
 }
@@ -987,6 +1108,7 @@ extern "C" {
 int asm_get_rk_dens(asm_context* ctx, float* rk, float* dens, int32_t* igrp)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "no results before asm_tree_construct");
     HIPC(hipSetDevice(ctx->device));
     const int64_t n = ctx->n;
@@ -1010,6 +1132,7 @@ int asm_get_rk_dens(asm_context* ctx, float* rk, float* dens, int32_t* igrp)
 int asm_get_particles(asm_context* ctx, asm_particle* particles)
 {
     if (!ctx || !particles) return ASM_ERR_ARG;
+    SETTLE(ctx);
     const int64_t n = ctx->n;
     std::vector<float> rk(n), dens(n);
     std::vector<int32_t> igrp(n);
@@ -1027,6 +1150,7 @@ int asm_get_particles(asm_context* ctx, asm_particle* particles)
 int asm_get_neighbors(asm_context* ctx, int32_t* nbr)
 {
     if (!ctx || !nbr) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (!ctx->have_search || !ctx->keep_neighbors || !ctx->nbr.p)
         return fail(ctx, ASM_ERR_STATE, "neighbour lists not kept (set ASM_OPT_KEEP_NEIGHBORS before the search)");
     HIPC(hipSetDevice(ctx->device));
@@ -1040,6 +1164,7 @@ int asm_get_tree(asm_context* ctx, float* spos, int32_t* nodes, int64_t* nleaf, 
                  int32_t* leaf_size)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (!ctx->have_tree) return fail(ctx, ASM_ERR_STATE, "no tree before asm_tree_construct");
     HIPC(hipSetDevice(ctx->device));
     HIPC(hipStreamSynchronize(ctx->stream));
@@ -1056,6 +1181,7 @@ int asm_get_tree(asm_context* ctx, float* spos, int32_t* nodes, int64_t* nleaf, 
 int asm_get_sorted_results(asm_context* ctx, float* rk, float* dens)
 {
     if (!ctx) return ASM_ERR_ARG;
+    SETTLE(ctx);
     if (!ctx->have_search) return fail(ctx, ASM_ERR_STATE, "no search results");
     HIPC(hipSetDevice(ctx->device));
     HIPC(hipStreamSynchronize(ctx->stream));
@@ -1099,6 +1225,7 @@ int asm_reset_timers(asm_context* ctx)
 int asm_get_search_stats(asm_context* ctx, uint64_t out[4])
 {
     if (!ctx || !out) return ASM_ERR_ARG;
+    SETTLE(ctx);
     HIPC(hipSetDevice(ctx->device));
     out[0] = out[1] = out[2] = out[3] = 0;
     if (!ctx->have_search || !ctx->kstats.p) return ASM_OK;

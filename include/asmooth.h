@@ -134,7 +134,13 @@ int asm_mark_halo_particles(asm_context* ctx, const asm_halo* halos, int64_t n_h
 
 /* clear_mesh + assign_on_mesh (coarse_mesh.h:4,6; clumping_tree_main.cpp:525-526)
  * into the context's device mesh for `nc` (6*nc^3 floats: n | halo | c | v
- * with v xyz-interleaved).  sign as in the reference (-1 for the SO pass). */
+ * with v xyz-interleaved).  sign as in the reference (-1 for the SO pass).
+ * The first deposition of a context reads its tile-list size back to size
+ * the list buffer (with an eighth to spare); later ones (this call, the
+ * list and the periodic variant) return without waiting on the device: the
+ * next call that reads a mesh or changes the context's inputs or options
+ * redoes, before anything else, any deposition whose list outgrew the
+ * buffer, so results never depend on it. */
 int asm_assign_on_mesh(asm_context* ctx, int nc, int sign);
 /* Every nc of a -nc list in one call (clumping_tree_main.cpp:521-533). */
 int asm_assign_on_meshes(asm_context* ctx, const int* ncs, int n_nc, int sign);

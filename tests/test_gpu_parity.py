@@ -562,6 +562,48 @@ def test_spline_kernel_with_ties_matches_oracle(ctx, method):
         ctx.set_option(A.OPT_KERNEL, 0)
 
 
+def test_deposition_without_size_readback_redoes_what_did_not_fit():
+    """A context's later depositions read back no list size: each compares
+    its (particle, tile) total with the list buffer of an earlier call on the
+    device, and the next call that reads a mesh redoes, with the list sized
+    from its total, a deposition that did not fit.  A context sized on a small
+    set is given sets with more entries, for the nc list and for one mesh at a
+    time: the meshes equal the oracle, and the deposition stage's launch
+    counts show exactly the redos expected (two list stages per deposition)."""
+    ncs = [8, 16]
+    sets = [("small", small_set("blobs", 12, 21), True), ("big", small_set("zeldovich", 16, 22), True),
+            ("big again", small_set("zeldovich", 16, 22), True), ("bigger", small_set("blobs", 28, 23), False)]
+    c = A.Context(0)
+    try:
+        c.set_option(A.OPT_PROFILE, 1)
+        for i, (name, (xv, L, bf), multi) in enumerate(sets):
+            parts, nl = A.build_buffered_set(xv, L, bf)
+            buf = np.concatenate([parts["x"], parts["v"]], axis=1)
+            o = oracle_path(buf, L, ncs)
+            gpu_path(c, parts, nl, L, bf, [])
+            c.reset_timers()
+            if multi:
+                c.assign_on_meshes(ncs, -1)
+            else:
+                for nc in ncs:
+                    c.assign_on_mesh(nc, -1)
+            tn, _ = c.mesh_total(ncs[0])   # reads a mesh: settles the depositions
+            runs = c.timers()["deposit_lists"][1] // 2
+            # first call: sized by a read-back; the same set again: fits; a
+            # denser set (for one mesh at a time, denser than the previous
+            # list for either mesh): one redo per deposition call
+            calls = 1 if multi else len(ncs)
+            expect = calls if name in ("small", "big again") else 2 * calls
+            assert runs == expect, (name, runs)
+            vmax = float(np.max(np.abs(buf[:, 3:])))
+            for nc in ncs:
+                _check_mesh(c.get_mesh(nc), o["meshes"][nc], nc, vmax)
+            on, _ = O.mesh_total(o["meshes"][ncs[0]], ncs[0])
+            assert abs(tn - on) <= 1e-6 * abs(on)
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("ncs,tile", [((8, 16, 32), 8), ((1, 2, 3, 16), 5), ((12, 7, 12, 20), 3), ((24, 6), 8)])
 def test_all_meshes_in_one_pass_match_oracle(ctx, ncs, tile):
     """asm_assign_on_meshes: every nc of the list from one count pass, one fill
