@@ -683,18 +683,21 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
         // weight or field that is not finite (h = 0 gives infinite 1/h)
         // would turn the zeros into NaN; such batches take the walk that
         // tests every member's own box instead.
+        // The row is cleared whole (six 16-byte stores), then the entry's own
+        // cells are written over it: the loop runs over the own box only.
         bool bad = valid && !(isfinite(fe.x) && isfinite(fe.y) && isfinite(fe.z) && isfinite(fe.w));
+        {
+            float4* const row = reinterpret_cast<float4*>(W + lane * kColWStride);
+#pragma unroll
+            for (int k = 0; k < kColWStride / 4; ++k) row[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            for (int t = 0; t < un[d]; ++t) {
-                const int c = ulo[d] + t;
-                float w = 0.0f;
-                if (c >= lo[d] && c < hi[d]) {
-                    const float s1 = (static_cast<float>(c0[d] + c) * dxc - x[d]) * hinv;
-                    w = cell_integ(a.kern, s1, s1 + ds);
-                    bad = bad || !isfinite(w);
-                }
-                W[lane * kColWStride + d * kTileMax + t] = w;
+            for (int c = lo[d]; c < hi[d]; ++c) {
+                const float s1 = (static_cast<float>(c0[d] + c) * dxc - x[d]) * hinv;
+                const float w = cell_integ(a.kern, s1, s1 + ds);
+                bad = bad || !isfinite(w);
+                W[lane * kColWStride + d * kTileMax + (c - ulo[d])] = w;
             }
         const bool halo_batch = __any(valid && !igm);
         const bool clean_batch = !__any(bad);
