@@ -632,7 +632,7 @@ __global__ __launch_bounds__(kBoundThreads, 8) void knn_bound_kernel(
 // (they include every leaf within tau of some lane).  A wavefront whose
 // union would exceed kLeafCap leaves, or whose seed holds fewer than 33
 // particles, is flagged in `redo` and left to the packet walk.
-constexpr int kSelThreads = 256;
+constexpr int kSelThreads = 64;
 constexpr int kSelWaves = kSelThreads / kWave;
 constexpr int kSelBins = 64;                     // byte counters per lane
 constexpr int kSelHistWords = kSelBins / 4;
