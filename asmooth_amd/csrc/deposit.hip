@@ -606,6 +606,7 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
     __shared__ Acc acc[6][kPadCells];
     __shared__ __align__(16) float wts[kWaves][kWave * kColWStride];
     __shared__ int smap[kWaves][kWave];
+    __shared__ uint32_t next_batch;   // batches are handed out in turn to the wavefronts that ask
     const int tid = threadIdx.x;
     const int lane = tid % kWave;
     const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
@@ -620,6 +621,7 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
     const int cx0 = tx * T, cy0 = ty * T, cz0 = tz * T;
     for (int c = tid; c < 6 * kPadCells; c += kThreads) (&acc[0][0])[c] = Acc(0);
     map[lane] = -1;
+    if (tid == 0) next_batch = kWaves;
     __syncthreads();
 
     const float dxc = a.boxsize / static_cast<float>(a.nc);
@@ -628,16 +630,24 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
     const uint32_t ngrp = (count + G - 1) / G;
     const uint32_t nb = (ngrp + kGroups - 1) / kGroups;   // batches; batch b holds groups q * nb + b
     int tag = 0;                                          // sweep counter, tags the map marks
-    // The list entry of the wavefront's next batch is read one batch ahead,
-    // so that only the particle loads wait at the top of each batch.
+    // Wavefront w starts with batch w; each later batch goes to whichever
+    // wavefront asks first (batches differ in work), one batch ahead, so that
+    // the list entry of the wavefront's next batch is read while it works on
+    // this one and only the particle loads wait at the top of each batch.
     const auto entry_of = [&](uint32_t bb) { return (static_cast<uint32_t>(qi) * nb + bb) * G + g; };
+    const auto claim = [&]() {
+        uint32_t v = 0;
+        if (lane == 0) v = atomicAdd(&next_batch, 1u);
+        return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+    };
     uint32_t next_idx = 0;
     if (wid < nb && entry_of(wid) < count) next_idx = list[begin + entry_of(wid)];
-    for (uint32_t b = wid; b < nb; b += kWaves) {
+    uint32_t b_next = static_cast<uint32_t>(wid) < nb ? claim() : nb;
+    for (uint32_t b = wid; b < nb; b = b_next, b_next = b_next < nb ? claim() : nb) {
         const uint32_t e = entry_of(b);
         const bool valid = e < count;
         const uint32_t cur_idx = next_idx;
-        if (b + kWaves < nb && entry_of(b + kWaves) < count) next_idx = list[begin + entry_of(b + kWaves)];
+        if (b_next < nb && entry_of(b_next) < count) next_idx = list[begin + entry_of(b_next)];
         // The entry's box [lo, hi) of tile cells per axis (empty: lo 8, hi 0).
         int lo[3] = {kTileMax, kTileMax, kTileMax}, hi[3] = {0, 0, 0};
         float x[3] = {0.0f, 0.0f, 0.0f};
