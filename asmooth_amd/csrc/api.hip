@@ -135,7 +135,7 @@ struct asm_context {
     // learned seed radius
     bool seed_on = false;
     float seed_margin = 0.0f;
-    int seed_walk = 0;   // ASM_OPT_SEED_WALK
+    int seed_walk = 2;   // ASM_OPT_SEED_WALK
     DevBuf<float> seed_params, seed_grid, seed_rcell, tau_init;
     DevBuf<uint32_t> seed_redo;
     DevBuf<unsigned long long> seed_fb;
@@ -352,7 +352,10 @@ int asm_set_option(asm_context* ctx, int option, int value)
         ctx->have_search = false;
         return ASM_OK;
     case ASM_OPT_DETERMINISTIC: ctx->deterministic = value ? 1 : 0; return ASM_OK;
-    case ASM_OPT_SEED_WALK: ctx->seed_walk = value ? 1 : 0; return ASM_OK;
+    case ASM_OPT_SEED_WALK:
+        if (value < 0 || value > 2) return fail(ctx, ASM_ERR_ARG, "ASM_OPT_SEED_WALK %d outside 0..2", value);
+        ctx->seed_walk = value;
+        return ASM_OK;
     case ASM_OPT_KNN_METHOD:
         if (value != kKnnBoundCollect && value != kKnnInsertion)
             return fail(ctx, ASM_ERR_ARG, "kNN method %d is not 0 or 1", value);

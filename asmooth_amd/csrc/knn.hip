@@ -287,7 +287,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_dens_kernel(
 // ---------------------------------------------------------------------------
 // Bound walk and collect walk.
 
-constexpr int kBoundThreads = 128;
+constexpr int kBoundThreads = 64;
 constexpr int kBoundWaves = kBoundThreads / kWave;
 constexpr int kCollectThreads = 64;
 constexpr int kCollectWaves = kCollectThreads / kWave;
@@ -748,7 +748,9 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     int leaf_size, int init_leaves, float* __restrict__ tau_out, uint32_t* __restrict__ wleaf,
     uint32_t* __restrict__ wleaf_n, uint32_t* __restrict__ redo, KnnStats* __restrict__ stats,
     const Box* __restrict__ groups, uint32_t* __restrict__ list, uint32_t* __restrict__ list_n,
-    float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted)
+    float* __restrict__ rk_sorted, uint64_t* __restrict__ kkey_sorted, const float* __restrict__ tau_init,
+    const uint32_t* __restrict__ only_flagged, uint32_t* __restrict__ seed_redo,
+    unsigned long long* __restrict__ seed_fallbacks)
 {
     static_assert(K == 33, "the selection takes the 33rd smallest of 64 (wave_33rd)");
     __shared__ SelectLds lds[kSelWaves];
@@ -758,6 +760,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     const int64_t wave0 = (static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kSelWaves + wid) * kWave;
     if (wave0 >= n) return;
     const int64_t wave = wave0 / kWave;
+    if (only_flagged && only_flagged[wave] == 0u) return;
     // Particle, leaf and node indices fit in 32 bits (the launcher checks n).
     const int ni = static_cast<int>(n);
     const int w0 = static_cast<int>(wave0);
@@ -774,66 +777,85 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     int l1 = last / leaf_size + init_leaves;
     if (l0 < 0) l0 = 0;
     if (l1 > static_cast<int>(nleaf) - 1) l1 = static_cast<int>(nleaf) - 1;
-    // Seed of the bound: the wavefront's particles and kSelSeedHalo on
-    // either side in Morton order (at most 254 particles).
-    const int s0 = w0 - kSelSeedHalo > 0 ? w0 - kSelSeedHalo : 0;
-    const int s1 = w0 + kWave + kSelSeedHalo < ni ? w0 + kWave + kSelSeedHalo : ni;
-    if (s1 - s0 < K) {
-        if (lane == 0) redo[wave] = 1u;
-        return;
-    }
-
-    // ---- A: per-lane bound from the seed range --------------------------
-    float blo0 = INFINITY, blo1 = INFINITY, blo2 = INFINITY;
-    float bhi0 = -INFINITY, bhi1 = -INFINITY, bhi2 = -INFINITY;
-    for (int l = s0 / leaf_size; l <= (s1 - 1) / leaf_size; ++l) {
-        const Box b = nodes[nlp + l];
-        blo0 = fminf(blo0, b.lo[0]);
-        blo1 = fminf(blo1, b.lo[1]);
-        blo2 = fminf(blo2, b.lo[2]);
-        bhi0 = fmaxf(bhi0, b.hi[0]);
-        bhi1 = fmaxf(bhi1, b.hi[1]);
-        bhi2 = fmaxf(bhi2, b.hi[2]);
-    }
-    // Every squared distance within the seed is at most the box
-    // diagonal's (plus rounding, under one bin step): bins end at its bin + 1.
-    const float diag2 = norm2(bhi0, bhi1, bhi2, blo0, blo1, blo2);
-    const int base = static_cast<int>(__float_as_uint(diag2) >> kSelBinShift) + 1 - (kSelBins - 1);
-#pragma unroll
-    for (int w = 0; w < kSelHistWords; ++w) S.a.hist[w][lane] = 0u;
-    float dmax = 0.0f;
-    for (int b0 = s0; b0 < s1; b0 += kWave) {
-        const int m = s1 - b0 < kWave ? s1 - b0 : kWave;
-        S.a.stage[lane] = lane < m ? spos[b0 + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        wave_lds_fence();
-        for (int j = 0; j < m; ++j) {
-            const float4 c = S.a.stage[j];
-            const float d2 = norm2(me.x, me.y, me.z, c.x, c.y, c.z);
-            dmax = fmaxf(dmax, d2);
-            const int k = static_cast<int>(__float_as_uint(d2) >> kSelBinShift) - base;
-            const int bin = k < 0 ? 0 : (k > kSelBins - 1 ? kSelBins - 1 : k);
-            atomicAdd(&S.a.hist[bin >> 2][lane], 1u << ((bin & 3) * 8));
+    // With learned bounds (tau_init, an upper estimate of each particle's
+    // 33rd squared distance) the selection runs in two passes.  The first
+    // (only_flagged null) skips phase A and bounds every query by its learned
+    // bound; a query whose bound holds fewer than 33 particles is left
+    // unsettled (tau_out -1) and flags its wavefront in seed_redo.  The second
+    // (only_flagged = seed_redo) settles those queries with the seed-range
+    // bound; the wavefront's other queries keep their results and bound the
+    // union by the smaller of their two bounds.
+    const bool learned = tau_init && !only_flagged;
+    const bool redo_pass = tau_init && only_flagged;
+    const bool need = valid && (!redo_pass || tau_out[q] < 0.0f);   // this pass settles the query
+    float rA2;
+    int n_seed = 0;   // particles of the seed range, whose distances phase A evaluated
+    if (learned) {
+        rA2 = tau_init[valid ? q : ni - 1];
+        if (lane == 0) seed_redo[wave] = 0u;
+    } else {
+        // Seed of the bound: the wavefront's particles and kSelSeedHalo on
+        // either side in Morton order (at most 254 particles).
+        const int s0 = w0 - kSelSeedHalo > 0 ? w0 - kSelSeedHalo : 0;
+        const int s1 = w0 + kWave + kSelSeedHalo < ni ? w0 + kWave + kSelSeedHalo : ni;
+        if (s1 - s0 < K) {
+            if (lane == 0) redo[wave] = 1u;
+            return;
         }
-        wave_lds_fence();
-    }
-    int cum = 0, bsel = kSelBins - 1;
-    bool found = false;
-#pragma unroll
-    for (int w = 0; w < kSelHistWords; ++w) {
-        const uint32_t h = S.a.hist[w][lane];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            cum += static_cast<int>((h >> (8 * t)) & 0xffu);
-            if (!found && cum >= K) {
-                found = true;
-                bsel = 4 * w + t;
+        n_seed = s1 - s0;
+
+        // ---- A: per-lane bound from the seed range --------------------------
+        float blo0 = INFINITY, blo1 = INFINITY, blo2 = INFINITY;
+        float bhi0 = -INFINITY, bhi1 = -INFINITY, bhi2 = -INFINITY;
+        for (int l = s0 / leaf_size; l <= (s1 - 1) / leaf_size; ++l) {
+            const Box b = nodes[nlp + l];
+            blo0 = fminf(blo0, b.lo[0]);
+            blo1 = fminf(blo1, b.lo[1]);
+            blo2 = fminf(blo2, b.lo[2]);
+            bhi0 = fmaxf(bhi0, b.hi[0]);
+            bhi1 = fmaxf(bhi1, b.hi[1]);
+            bhi2 = fmaxf(bhi2, b.hi[2]);
+        }
+        // Every squared distance within the seed is at most the box
+        // diagonal's (plus rounding, under one bin step): bins end at its bin + 1.
+        const float diag2 = norm2(bhi0, bhi1, bhi2, blo0, blo1, blo2);
+        const int base = static_cast<int>(__float_as_uint(diag2) >> kSelBinShift) + 1 - (kSelBins - 1);
+    #pragma unroll
+        for (int w = 0; w < kSelHistWords; ++w) S.a.hist[w][lane] = 0u;
+        float dmax = 0.0f;
+        for (int b0 = s0; b0 < s1; b0 += kWave) {
+            const int m = s1 - b0 < kWave ? s1 - b0 : kWave;
+            S.a.stage[lane] = lane < m ? spos[b0 + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            wave_lds_fence();
+            for (int j = 0; j < m; ++j) {
+                const float4 c = S.a.stage[j];
+                const float d2 = norm2(me.x, me.y, me.z, c.x, c.y, c.z);
+                dmax = fmaxf(dmax, d2);
+                const int k = static_cast<int>(__float_as_uint(d2) >> kSelBinShift) - base;
+                const int bin = k < 0 ? 0 : (k > kSelBins - 1 ? kSelBins - 1 : k);
+                atomicAdd(&S.a.hist[bin >> 2][lane], 1u << ((bin & 3) * 8));
+            }
+            wave_lds_fence();
+        }
+        int cum = 0, bsel = kSelBins - 1;
+        bool found = false;
+    #pragma unroll
+        for (int w = 0; w < kSelHistWords; ++w) {
+            const uint32_t h = S.a.hist[w][lane];
+    #pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                cum += static_cast<int>((h >> (8 * t)) & 0xffu);
+                if (!found && cum >= K) {
+                    found = true;
+                    bsel = 4 * w + t;
+                }
             }
         }
+        // The top bin also holds anything clamped into it: its bound is the
+        // largest squared distance seen.
+        rA2 = bsel == kSelBins - 1 ? dmax : __uint_as_float(static_cast<uint32_t>(base + bsel + 1) << kSelBinShift);
+        if (redo_pass && !need) rA2 = fminf(rA2, tau_init[valid ? q : ni - 1]);
     }
-    // The top bin also holds anything clamped into it: its bound is the
-    // largest squared distance seen.
-    const float rA2 = bsel == kSelBins - 1 ? dmax
-                                           : __uint_as_float(static_cast<uint32_t>(base + bsel + 1) << kSelBinShift);
     wave_lds_fence();   // the histogram's LDS is reused below
 
     // ---- B: union of the leaves within some lane's bound ----------------
@@ -1033,7 +1055,10 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
     float tau = 0.0f;
     uint32_t my_n = 0;          // with lists: lane i's list length ...
     uint32_t my_p33 = 0;        // ... and the sorted position of its 33rd key's particle
+    const uint64_t need_mask = wave_ballot(need);
+    uint64_t fail_mask = 0ull;   // first pass: queries whose learned bound held fewer than 33
     for (int i = 0; i < nvalid; ++i) {
+        if (redo_pass && ((need_mask >> i) & 1ull) == 0ull) continue;
         float4 qq;
         qq.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.x), i));
         qq.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(me.y), i));
@@ -1177,6 +1202,10 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
                 }
             }
         }
+        if (learned && cc < K) {
+            fail_mask |= 1ull << i;   // settled by the second pass
+            continue;
+        }
         const uint32_t x = lane < cc ? cand[lane] : 0xffffffffu;
         const uint32_t tb = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(wave_33rd(x, lane))));
         const float ti = __uint_as_float(tb);
@@ -1221,12 +1250,20 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
         }
         wave_lds_fence();
     }
-    if (valid) {
+    const bool failed = ((fail_mask >> lane) & 1ull) != 0ull;
+    if (need && !failed) {
         tau_out[q] = tau;
         if (list) {
             list_n[q] = my_n;
             rk_sorted[q] = sqrtf(tau);
             kkey_sorted[q] = make_key(tau, __float_as_uint(spos[my_p33].w));
+        }
+    }
+    if (learned) {
+        if (failed) tau_out[q] = -1.0f;
+        if (lane == 0 && fail_mask != 0ull) {
+            seed_redo[wave] = 1u;
+            atomicAdd(&seed_fallbacks[wave & (kStatSlots - 1)], static_cast<unsigned long long>(__popcll(fail_mask)));
         }
     }
 
@@ -1249,7 +1286,7 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(8))
         KnnStats* const st = stats + (wave & (kStatSlots - 1));
         atomicAdd(&st->node_tests, static_cast<unsigned long long>(n_node));
         atomicAdd(&st->leaf_scans, static_cast<unsigned long long>(nu));
-        atomicAdd(&st->dist_evals, static_cast<unsigned long long>(s1 - s0) * static_cast<unsigned long long>(nvalid) +
+        atomicAdd(&st->dist_evals, static_cast<unsigned long long>(n_seed) * static_cast<unsigned long long>(nvalid) +
                                        static_cast<unsigned long long>(n_eval));
         atomicAdd(&st->inserts, static_cast<unsigned long long>(n_acc));
     }
@@ -1870,10 +1907,12 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
     const unsigned col_blocks = static_cast<unsigned>((waves + kCollectWaves - 1) / kCollectWaves);
     // The selection walk lists each particle's neighbours when no ordered
     // lists are kept; the collect walk then runs only where it could not.
-    const bool lists = !scratch.tau_init && scratch.bound == kBoundSelect && groups && scratch.list &&
-                       scratch.list_n && !nbr_out_orig;
+    // The selection walk runs without a seed model, or with one when the
+    // learned bounds are to bound it (seed_walk 2).
+    const bool select = scratch.bound == kBoundSelect && groups && (!scratch.tau_init || scratch.seed_walk == 2);
+    const bool lists = select && scratch.list && scratch.list_n && !nbr_out_orig;
     mark_begin(0);
-    if (scratch.tau_init && scratch.seed_walk) {
+    if (scratch.tau_init && scratch.seed_walk == 1) {
 #define ASM_SEED_WALK(PAD)                                                                                        \
     knn_seed_walk_kernel<kSlots, PAD><<<col_blocks, kCollectThreads, 0, s>>>(                                     \
         spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau_init, scratch.tau, scratch.wleaf,   \
@@ -1883,12 +1922,22 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
         else if (leaf_size <= 32) ASM_SEED_WALK(32);
         else ASM_SEED_WALK(64);
 #undef ASM_SEED_WALK
-    } else if (!scratch.tau_init && scratch.bound == kBoundSelect && groups) {
+    } else if (select) {
         const unsigned sel_blocks = static_cast<unsigned>((waves + kSelWaves - 1) / kSelWaves);
         knn_select_kernel<kSlots><<<sel_blocks, kSelThreads, 0, s>>>(
             spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,
-            scratch.wave_flags, stats, groups, lists ? scratch.list : nullptr, scratch.list_n, rk_sorted, kkey_sorted);
+            scratch.wave_flags, stats, groups, lists ? scratch.list : nullptr, scratch.list_n, rk_sorted, kkey_sorted,
+            scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks);
         ASM_HIP_CHECK(hipGetLastError());
+        if (scratch.tau_init) {
+            // Wavefronts with a query whose learned bound held fewer than 33:
+            // those queries again, with the seed-range bound.
+            knn_select_kernel<kSlots><<<sel_blocks, kSelThreads, 0, s>>>(
+                spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf,
+                scratch.wleaf_n, scratch.wave_flags, stats, groups, lists ? scratch.list : nullptr, scratch.list_n,
+                rk_sorted, kkey_sorted, scratch.tau_init, scratch.seed_redo, nullptr, nullptr);
+            ASM_HIP_CHECK(hipGetLastError());
+        }
         // Wavefronts the selection flagged (union past kLeafCap leaves, or a
         // seed range under 33 particles): the packet walk; every other
         // wavefront returns at once.  The collect walk rewrites the flags.
@@ -1901,7 +1950,7 @@ hipError_t launch_knn(const float4* spos, int64_t n, const Box* nodes, int64_t n
             stats, scratch.tau_init, nullptr, scratch.seed_redo, scratch.seed_fallbacks, groups);
     }
     ASM_HIP_CHECK(hipGetLastError());
-    if (scratch.tau_init) {
+    if (scratch.tau_init && !select) {
         // Wavefronts with a lane whose learned bound was too small: again, unbounded.
         knn_bound_kernel<kSlots><<<bound_blocks, kBoundThreads, 0, s>>>(
             spos, n, nodes, nleaf_pad, nleaf, leaf_size, init_leaves, scratch.tau, scratch.wleaf, scratch.wleaf_n,

@@ -104,10 +104,13 @@ def run_step(ctx, L: float, bf: float, ncs):
     return totals
 
 
-def learned_seed_leg(ctx, parts, L: float, bf: float, ncs, steps: int, plain_ms: float, plain_rk, margin: float):
+def learned_seed_leg(ctx, parts, L: float, bf: float, ncs, steps: int, plain_ms: float, plain_knn_ms: float,
+                     plain_rk, margin: float):
     """The same workload with the learned seed radius (asmooth_amd/seed_mlp.npy,
     trained on Zel'dovich sets of seeds 101-108; this workload's seed 1 is
-    held out): per-step time, fallback rate, and h compared bit for bit."""
+    held out) bounding the selection walk: per-step time, the kNN stage's
+    gain (its time with the seed stage against the plain kNN stage's),
+    fallback rates, and h compared bit for bit."""
     import asmooth_amd as A
     from asmooth_amd.learned_seed import DEFAULT_MODEL
     if not os.path.exists(DEFAULT_MODEL):
@@ -128,12 +131,15 @@ def learned_seed_leg(ctx, parts, L: float, bf: float, ncs, steps: int, plain_ms:
     ctx.set_seed_model(None)
     model.close()
     nl = ctx.np_local
+    stage = {k: round(v[0] / steps, 4) for k, v in timers.items() if v[1]}
+    knn_ms = stage.get("knn", 0.0) + stage.get("seed", 0.0)
     return {"value": round(nl / (ms * 1e-3), 1), "unit": "particles/s", "ms_per_step": round(ms, 3),
-            "gain_vs_plain": round(plain_ms / ms, 4), "margin": margin,
+            "gain_vs_plain": round(plain_ms / ms, 4),
+            "knn_stage_gain": round(plain_knn_ms / knn_ms, 4) if knn_ms > 0 else None, "margin": margin,
             "fallback_rate": round(st["fallback_particles"] / st["particles"], 6),
             "fallback_wavefront_rate": round(st["fallback_wavefronts"] * 64 / st["particles"], 6),
             "h_bit_exact_vs_plain": bool(np.array_equal(rk.view(np.int32), plain_rk.view(np.int32))),
-            "stage_ms": {k: round(v[0] / steps, 4) for k, v in timers.items() if v[1]}}
+            "stage_ms": stage}
 
 
 def batched_leg(steps: int, n_streams: int = 8):
@@ -268,7 +274,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
-    ap.add_argument("--seed-margin", type=float, default=0.3)
+    ap.add_argument("--seed-margin", type=float, default=0.15)
     ap.add_argument("--batch-streams", type=int, default=8, help="streams the batched graph forks into")
     ap.add_argument("--leaf-size", type=int, default=32, help="particles per tree leaf (8..64)")
     ap.add_argument("--tile", type=int, default=8, help="deposition tile side in cells (1..8)")
@@ -362,10 +368,12 @@ def main(argv=None) -> int:
     learned = None
     if not args.no_learned:
         plain_rk, _, _ = ctx.get_rk_dens()
-        learned = learned_seed_leg(ctx, parts, L, bf, ncs, max(2, min(args.steps, 3)), ms_per_step, plain_rk,
-                                   args.seed_margin)
+        plain_knn_ms = timers["knn"][0] / max(timers["knn"][1], 1)
+        learned = learned_seed_leg(ctx, parts, L, bf, ncs, max(2, min(args.steps, 3)), ms_per_step, plain_knn_ms,
+                                   plain_rk, args.seed_margin)
         if learned:
             log(f"[bench] learned seed: {learned['ms_per_step']} ms/step, gain {learned['gain_vs_plain']}, "
+                f"kNN stage gain {learned['knn_stage_gain']}, "
                 f"fallback {learned['fallback_rate']}, h bit-exact {learned['h_bit_exact_vs_plain']}")
     ctx.close()
     batched = None

@@ -72,9 +72,12 @@ enum {
                                       insertion lists for every wavefront.  Same results. */
     ASM_OPT_DEPOSIT_ACCUM = 7,  /* node-mesh cell sums: 0 FP64 (default); 1 FP32 with Kahan
                                       compensation; 2 plain FP32 atomics (precision study) */
-    ASM_OPT_SEED_WALK = 8,      /* with a seed model: 0 (default) starts the bounded walk of
-                                      the plain search from the learned bounds; 1 searches within
-                                      them by a range walk (a study: slower on the 256^3
+    ASM_OPT_SEED_WALK = 8,      /* with a seed model: 2 (default) bounds the selection walk by
+                                      the learned bounds instead of its seed-range histogram,
+                                      settling the queries whose bound holds fewer than 33
+                                      particles in a second pass with the histogram bound; 0
+                                      starts the packet walk from the learned bounds; 1 searches
+                                      within them by a range walk (studies: slower on the 256^3
                                       workload).  Results are the same. */
     ASM_OPT_KERNEL = 9,         /* smoothing kernel: 0 (default) the reference's separable tent
                                       (ker, nbr_search.cpp:27-37; coarse_mesh.cpp:10-22); 1 the

@@ -54,12 +54,13 @@ def _const_model(b3):
     return A.Mlp(0, p)
 
 
-@pytest.mark.parametrize("walk", [1, 0])
+@pytest.mark.parametrize("walk", [2, 1, 0])
 @pytest.mark.parametrize("b3,margin", [(4.0, 0.0), (-4.0, 0.0), (0.55, 0.05), (0.2, 0.3)])
 def test_seeded_search_is_exact(ctx, b3, margin, walk):
     # radius s * e^b3 * (1 + margin): far too large, far too small (every
     # particle searched again), and near the typical h (some searched again);
-    # walk 1: range walk within the bounds, walk 0: bounded walk
+    # walk 2: the selection walk within the bounds, 1: range walk within the
+    # bounds, 0: bounded packet walk
     xv, L, bf = small_set("zeldovich", 20, 9)
     parts, nl = A.build_buffered_set(xv, L, bf)
     plain = gpu_path(ctx, parts, nl, L, bf, [10])
@@ -71,7 +72,7 @@ def test_seeded_search_is_exact(ctx, b3, margin, walk):
         st = ctx.seed_stats()
     finally:
         ctx.set_seed_model(None)
-        ctx.set_option(A.OPT_SEED_WALK, 0)
+        ctx.set_option(A.OPT_SEED_WALK, 2)
     assert np.array_equal(seeded["rk"].view(np.int32), plain["rk"].view(np.int32))
     assert np.array_equal(seeded["nbr"], plain["nbr"])
     assert np.array_equal(seeded["dens"], plain["dens"]) or \
@@ -95,7 +96,7 @@ def test_seeded_range_walk_leaf_sizes(ctx, leaf):
         st = ctx.seed_stats()
     finally:
         ctx.set_seed_model(None)
-        ctx.set_option(A.OPT_SEED_WALK, 0)
+        ctx.set_option(A.OPT_SEED_WALK, 2)
     assert np.array_equal(seeded["rk"].view(np.int32), plain["rk"].view(np.int32))
     assert np.array_equal(seeded["nbr"], plain["nbr"])
     assert np.max(np.abs(seeded["dens"] - plain["dens"]) / plain["dens"]) < 1e-6
