@@ -265,126 +265,139 @@ __global__ void adam_kernel(float* __restrict__ params, const float* __restrict_
 // Features on the smoothing context's sorted particles.
 
 // Cloud-in-cell counts on an nf^3 mesh of the cube [left, left + nf/inv).
-// Particles arrive in Morton order, so the 1024 of a workgroup cover a
-// small block of cells: the workgroup accumulates into an LDS window of
-// kCicWin^3 cells placed at the block's lowest stencil corner and adds the
-// window's non-zero cells to the mesh; stencils outside the window (a block
-// spanning a Morton jump) go to the mesh directly.
-// CIC counts on the kMlpGrid^3 mesh.  A workgroup takes kCicThreads x
-// kCicRun Morton-consecutive particles, which fall in a few cells: it adds
-// into an LDS window of kCicWin^3 cells at the smallest cell it touches,
-// then adds the window's nonzero cells to the mesh (global atomics).  Each
-// thread walks kCicRun consecutive particles and keeps the eight corner
-// weights of the current cell in registers until the cell changes (tens of
-// consecutive particles share a cell), so the LDS sees one set of eight
-// atomics per run of a cell instead of one per particle.  Corners outside
-// the window go straight to the mesh; corners past the mesh are clamped onto
-// its edge cells.
+// A workgroup takes kCicThreads x kCicRun Morton-consecutive particles.  It
+// reads them coalesced into LDS (mesh coordinates minus half a cell, one
+// array per axis), then each lane takes kCicRun consecutive ones: these
+// mostly share the stencil's lower corner (the particles of one octant of a
+// cell are consecutive in Morton order), so the lane sums their eight
+// weights in registers and adds the sums only when the corner changes, the
+// last sums of the lanes sharing a corner first summed across lanes.  The
+// sums go into an LDS window of kCicWin^3 cells (FP64, ds_add_f64) placed at
+// the lowest corner of the workgroup, whose non-zero cells are then added to
+// the mesh (FP32 global atomics); stencils leaving the window go straight to
+// the mesh.  Corners past the mesh are clamped onto its edge.
 constexpr int kCicThreads = 256;
-constexpr int kCicRun = 16;
+constexpr int kCicRun = 8;
 constexpr int kCicWin = 12;
+constexpr int kCicStage = kCicThreads * kCicRun + kCicThreads;   // one pad word per run: conflict-free reads
 
-__device__ __forceinline__ void cic_stencil(float4 p, float left, float inv_cell, int (&c0)[3], float (&w1)[3])
+__device__ __forceinline__ int cic_slot(int j) { return j + j / kCicRun; }
+
+__device__ __forceinline__ void cic_add(double* win, const int (&k0)[3], const int (&c)[3], const float (&acc)[8],
+                                        int nf, float* __restrict__ grid)
 {
-    const float u[3] = {(p.x - left) * inv_cell - 0.5f, (p.y - left) * inv_cell - 0.5f, (p.z - left) * inv_cell - 0.5f};
+    const bool inwin = c[0] >= k0[0] && c[1] >= k0[1] && c[2] >= k0[2] && c[0] + 1 - k0[0] < kCicWin &&
+                       c[1] + 1 - k0[1] < kCicWin && c[2] + 1 - k0[2] < kCicWin;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const float f = floorf(u[d]);
-        c0[d] = static_cast<int>(f);
-        w1[d] = u[d] - f;
+    for (int q = 0; q < 8; ++q) {
+        const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
+        if (inwin) {
+            atomicAdd(&win[((c[2] + dz - k0[2]) * kCicWin + (c[1] + dy - k0[1])) * kCicWin + (c[0] + dx - k0[0])],
+                      static_cast<double>(acc[q]));
+        } else {
+            const int cx = min(max(c[0] + dx, 0), nf - 1);
+            const int cy = min(max(c[1] + dy, 0), nf - 1);
+            const int cz = min(max(c[2] + dz, 0), nf - 1);
+            atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], acc[q]);
+        }
     }
-}
-
-// The eight corner weights of stencil w1 (x first), added to acc.
-__device__ __forceinline__ void cic_add(const float (&w1)[3], float (&acc)[8])
-{
-#pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx)
-                acc[(dz * 2 + dy) * 2 + dx] += (dx ? w1[0] : 1.0f - w1[0]) * (dy ? w1[1] : 1.0f - w1[1]) *
-                                               (dz ? w1[2] : 1.0f - w1[2]);
-}
-
-// The corners of cell c0 with weights acc: into the window when they fit,
-// else into the mesh.
-__device__ __forceinline__ void cic_flush(const int (&c0)[3], const float (&acc)[8], const int (&k0)[3], float* win,
-                                          int nf, float* __restrict__ grid)
-{
-    const bool inwin = c0[0] >= k0[0] && c0[1] >= k0[1] && c0[2] >= k0[2] && c0[0] + 1 - k0[0] < kCicWin &&
-                       c0[1] + 1 - k0[1] < kCicWin && c0[2] + 1 - k0[2] < kCicWin;
-#pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                const float w = acc[(dz * 2 + dy) * 2 + dx];
-                if (inwin) {
-                    atomicAdd(&win[((c0[2] + dz - k0[2]) * kCicWin + (c0[1] + dy - k0[1])) * kCicWin +
-                                   (c0[0] + dx - k0[0])],
-                              w);
-                } else {
-                    const int cx = min(max(c0[0] + dx, 0), nf - 1);
-                    const int cy = min(max(c0[1] + dy, 0), nf - 1);
-                    const int cz = min(max(c0[2] + dz, 0), nf - 1);
-                    atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], w);
-                }
-            }
 }
 
 __global__ __launch_bounds__(kCicThreads) void cic_kernel(const float4* __restrict__ spos, int64_t n, float left,
                                                           float inv_cell, int nf, float* __restrict__ grid)
 {
-    __shared__ float win[kCicWin * kCicWin * kCicWin];
+    __shared__ double win[kCicWin * kCicWin * kCicWin];
+    __shared__ float stage[3][kCicStage];
     __shared__ int corner[3];
-    const int64_t first = (static_cast<int64_t>(blockIdx.x) * kCicThreads + threadIdx.x) * kCicRun;
-    const int cnt = static_cast<int>(first >= n ? 0 : (n - first < kCicRun ? n - first : kCicRun));
-    for (int c = threadIdx.x; c < kCicWin * kCicWin * kCicWin; c += kCicThreads) win[c] = 0.0f;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kCicThreads * kCicRun;
+    const int64_t left_n = n - first;
+    const int cnt = left_n < kCicThreads * kCicRun ? static_cast<int>(left_n) : kCicThreads * kCicRun;
+    for (int c = threadIdx.x; c < kCicWin * kCicWin * kCicWin; c += kCicThreads) win[c] = 0.0;
     if (threadIdx.x < 3) corner[threadIdx.x] = 0x7fffffff;
     __syncthreads();
-    // Window corner: the smallest cell any particle of the workgroup touches.
+    // Indices clamped into the workgroup's particles and no branch, so that
+    // the loads are in flight together; slots past the count hold copies of
+    // the last particle and are never read.
+    float4 pv[kCicRun];
+#pragma unroll
+    for (int r = 0; r < kCicRun; ++r) pv[r] = spos[first + min(r * kCicThreads + static_cast<int>(threadIdx.x), cnt - 1)];
     int m0[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
-    for (int r = 0; r < cnt; ++r) {
-        int c0[3];
-        float w1[3];
-        cic_stencil(spos[first + r], left, inv_cell, c0, w1);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) m0[d] = min(m0[d], c0[d]);
+    for (int r = 0; r < kCicRun; ++r) {
+        const int j = r * kCicThreads + threadIdx.x;
+        const float4 p = pv[r];
+        const float u[3] = {(p.x - left) * inv_cell - 0.5f, (p.y - left) * inv_cell - 0.5f,
+                            (p.z - left) * inv_cell - 0.5f};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            stage[d][cic_slot(j)] = u[d];
+            m0[d] = min(m0[d], static_cast<int>(floorf(u[d])));
+        }
     }
-    if (cnt > 0)
+    // Minimum over the wavefront by shuffles, then one LDS atomic per
+    // wavefront (per-lane LDS atomics on one word serialise).
 #pragma unroll
-        for (int d = 0; d < 3; ++d) atomicMin(&corner[d], m0[d]);
+    for (int d = 0; d < 3; ++d) {
+        int v = m0[d];
+#pragma unroll
+        for (int o = kWave / 2; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o));
+        if (threadIdx.x % kWave == 0) atomicMin(&corner[d], v);
+    }
     __syncthreads();
     const int k0[3] = {corner[0], corner[1], corner[2]};
-    int cur[3] = {0, 0, 0};
-    float acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
-    for (int r = 0; r < cnt; ++r) {
+    int cur[3] = {0x7fffffff, 0, 0};
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int r = 0; r < kCicRun; ++r) {
+        const int j = threadIdx.x * kCicRun + r;
+        if (j >= cnt) break;
         int c0[3];
         float w1[3];
-        cic_stencil(spos[first + r], left, inv_cell, c0, w1);
-        if (r > 0 && (c0[0] != cur[0] || c0[1] != cur[1] || c0[2] != cur[2])) {
-            cic_flush(cur, acc, k0, win, nf, grid);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+        for (int d = 0; d < 3; ++d) {
+            const float u = stage[d][cic_slot(j)];
+            const float f = floorf(u);
+            c0[d] = static_cast<int>(f);
+            w1[d] = u - f;
+        }
+        if (c0[0] != cur[0] || c0[1] != cur[1] || c0[2] != cur[2]) {
+            if (cur[0] != 0x7fffffff) cic_add(win, k0, cur, acc, nf, grid);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) cur[d] = c0[d];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] = 0.0f;
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cur[d] = c0[d];
-        cic_add(w1, acc);
+        for (int q = 0; q < 8; ++q)
+            acc[q] += ((q & 1) ? w1[0] : 1.0f - w1[0]) * ((q & 2) ? w1[1] : 1.0f - w1[1]) *
+                      ((q & 4) ? w1[2] : 1.0f - w1[2]);
     }
-    if (cnt > 0) cic_flush(cur, acc, k0, win, nf, grid);
+    // Lanes whose runs end on the same corner are adjacent: sum each such
+    // segment of lanes into its first lane (suffix sums within segments), so
+    // one lane adds per corner instead of all of them onto the same words.
+    const int lane = threadIdx.x % kWave;
+    const bool head = lane == 0 || cur[0] != __shfl_up(cur[0], 1) || cur[1] != __shfl_up(cur[1], 1) ||
+                      cur[2] != __shfl_up(cur[2], 1);
+    const uint64_t heads = __ballot(head);
+    const int seg = __popcll(heads & (lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1)));
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int so = __shfl_down(seg, o);
+        const bool take = lane + o < kWave && so == seg;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float v = __shfl_down(acc[q], o);
+            if (take) acc[q] += v;
+        }
+    }
+    if (head && cur[0] != 0x7fffffff) cic_add(win, k0, cur, acc, nf, grid);
     __syncthreads();
     for (int c = threadIdx.x; c < kCicWin * kCicWin * kCicWin; c += kCicThreads) {
-        const float v = win[c];
-        if (v == 0.0f) continue;
+        const double v = win[c];
+        if (v == 0.0) continue;
         const int cx = min(max(k0[0] + c % kCicWin, 0), nf - 1);
         const int cy = min(max(k0[1] + (c / kCicWin) % kCicWin, 0), nf - 1);
         const int cz = min(max(k0[2] + c / (kCicWin * kCicWin), 0), nf - 1);
-        atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], v);
+        atomicAdd(&grid[(static_cast<int64_t>(cz) * nf + cy) * nf + cx], static_cast<float>(v));
     }
 }
 
