@@ -348,7 +348,7 @@ class Context:
         self._check(self._L.asm_seed_samples(self._h, _ptr(idx), len(idx), _ptr(x), _ptr(t)), "asm_seed_samples")
         return x, t
 
-    def set_seed_model(self, model: "Mlp | None", margin: float = 0.2) -> None:
+    def set_seed_model(self, model: "Mlp | None", margin: float = 0.15) -> None:
         self._check(self._L.asm_set_seed_model(self._h, model._h if model is not None else None, float(margin)),
                     "asm_set_seed_model")
 

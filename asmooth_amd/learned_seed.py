@@ -1,7 +1,7 @@
 """Training and evaluation of the learned seed radius.
 
     python -m asmooth_amd.learned_seed train [--runs 8] [--n 256] [--samples 500000] [--out PATH]
-    python -m asmooth_amd.learned_seed eval  [--n 512] [--nodes 2] [--seed 200] [--margins 0.2,0.3] [--model PATH]
+    python -m asmooth_amd.learned_seed eval  [--n 512] [--nodes 2] [--seed 200] [--margins 0.1,0.15,0.2,0.3] [--model PATH]
 
 `train` builds the training set from exact searches: for seeds 101.. (one
 per run) it generates the Zel'dovich recipe of the benchmark workload
