@@ -282,30 +282,82 @@ __global__ void deposit_count_kernel(DepositArgs a, uint32_t* __restrict__ tile_
 // small box of tiles: each (particle, tile) pair is counted in LDS, and one
 // global atomic per touched tile adds the workgroup's count (count pass) or
 // reserves the workgroup's slots of that tile's list (fill pass, then each
-// pair takes its slot with an LDS atomic).  A workgroup whose tile box
+// pair takes its slot within them).  A workgroup whose tile box
 // exceeds kListTable falls back to one global atomic per pair.  Both passes
 // see the same pairs, so the fill uses exactly the counted slots.
 constexpr int kListThreads = 256;
 constexpr int kListTable = 1024;
+// A wavefront whose particles touch at most this many tiles counts and fills
+// them by ballots (one LDS atomic per tile), instead of one LDS atomic per
+// (particle, tile) pair: Morton-consecutive particles share tiles, and
+// same-word LDS atomics serialise.
+constexpr int kListWaveTiles = 32;
+
+// A node-mesh particle's (x -+ h) / boxsize per axis: the part of its cell
+// range (axis_range, coarse_mesh.cpp:72-75) that does not depend on nc, so
+// computed once for every mesh of an nc list.
+struct ListPart {
+    float qlo[3], qhi[3];
+    bool valid;
+};
+
+__device__ __forceinline__ ListPart list_part(const DepositArgs& a, int64_t i)
+{
+    ListPart q;
+    q.valid = i < a.n;
+    const float4 p = a.spos[q.valid ? i : 0];
+    const float h = a.rk[q.valid ? i : 0];
+    const float x[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        q.qlo[d] = (x[d] - h) / a.boxsize;
+        q.qhi[d] = (x[d] + h) / a.boxsize;
+    }
+    return q;
+}
+
+// n / d for 0 <= n < 2^22 and d >= 1, from the reciprocal rounded to float
+// and one correction step (the product is within one of the quotient).
+__device__ __forceinline__ int small_div(int n, int d, float inv_d)
+{
+    int q = static_cast<int>(static_cast<float>(n) * inv_d);
+    if (q * d > n) --q;
+    else if ((q + 1) * d <= n) ++q;
+    return q;
+}
 
 // One mesh's pass of a workgroup; cnt, base and bb are the workgroup's LDS.
 // Every barrier inside is reached by all threads or by none (the early
 // returns are uniform, or taken where no barrier follows).
 template <bool kFill>
-__device__ __forceinline__ void node_list_body(const DepositArgs& a, uint32_t* __restrict__ counters,
+__device__ __forceinline__ void node_list_body(const DepositArgs& a, const ListPart& P,
+                                               uint32_t* __restrict__ counters,
                                                const uint32_t* __restrict__ tile_start, uint32_t* __restrict__ list,
                                                uint32_t* cnt, uint32_t* base, int* bb)
 {
     const int tid = threadIdx.x;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kListThreads + tid;
     int t0[3] = {0, 0, 0}, t1[3] = {-1, -1, -1};
-    if (i < a.n) {
-        const PartRange r = part_range(a, i);
-        if (r.active) {
+    if (P.valid) {
+        // The cell range of axis_range's node-mesh branch, from the shared
+        // quotients: the same operations in the same order.
+        const float ncf = static_cast<float>(a.nc);
+        const float inv_tile = 1.0f / static_cast<float>(a.tile);
+        bool act = true;
+        int lo[3], hi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int l = static_cast<int>(P.qlo[d] * ncf);
+            const int r = 1 + static_cast<int>(P.qhi[d] * ncf);
+            lo[d] = l > 0 ? l : 0;
+            hi[d] = r < a.nc ? r : a.nc;
+            if (lo[d] >= hi[d]) act = false;
+        }
+        if (act) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                t0[d] = r.lo[d] / a.tile;
-                t1[d] = (r.hi[d] - 1) / a.tile;
+                t0[d] = small_div(lo[d], a.tile, inv_tile);
+                t1[d] = small_div(hi[d] - 1, a.tile, inv_tile);
             }
         }
     }
@@ -333,6 +385,14 @@ __device__ __forceinline__ void node_list_body(const DepositArgs& a, uint32_t* _
             atomicMax(&bb[3 + d], v[3 + d]);
         }
     }
+    // The wavefront's own box of tiles (v is reduced over the wavefront, so
+    // uniform: held in scalar registers).
+    int wb[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) wb[d] = __builtin_amdgcn_readfirstlane(v[d]);
+    const int wvol = wb[3] >= 0 ? (wb[3] - wb[0] + 1) * (wb[4] - wb[1] + 1) * (wb[5] - wb[2] + 1) : 0;
+    const bool per_wave = wvol <= kListWaveTiles;
+    const int lane = tid % kWave;
     __syncthreads();
     if (bb[3] < 0) return;   // no active particle in the workgroup (uniform)
     const int bx = bb[3] - bb[0] + 1, by = bb[4] - bb[1] + 1, bz = bb[5] - bb[2] + 1;
@@ -357,11 +417,25 @@ __device__ __forceinline__ void node_list_body(const DepositArgs& a, uint32_t* _
     }
     for (int k = tid; k < vol; k += kListThreads) cnt[k] = 0u;
     __syncthreads();
-    if (active)
+    if (per_wave) {
+        // The lanes covering each tile of the wavefront's box, counted by
+        // one ballot and added by one lane.
+        for (int tz = wb[2]; tz <= wb[5]; ++tz)
+            for (int ty = wb[1]; ty <= wb[4]; ++ty)
+                for (int tx = wb[0]; tx <= wb[3]; ++tx) {
+                    const bool cov = active && tx >= t0[0] && tx <= t1[0] && ty >= t0[1] && ty <= t1[1] &&
+                                     tz >= t0[2] && tz <= t1[2];
+                    const uint64_t m = __ballot(cov);
+                    if (m != 0ull && lane == 0)
+                        atomicAdd(&cnt[((tz - bb[2]) * by + (ty - bb[1])) * bx + (tx - bb[0])],
+                                  static_cast<uint32_t>(__popcll(m)));
+                }
+    } else if (active) {
         for (int tz = t0[2]; tz <= t1[2]; ++tz)
             for (int ty = t0[1]; ty <= t1[1]; ++ty)
                 for (int tx = t0[0]; tx <= t1[0]; ++tx)
                     atomicAdd(&cnt[((tz - bb[2]) * by + (ty - bb[1])) * bx + (tx - bb[0])], 1u);
+    }
     __syncthreads();
     for (int k = tid; k < vol; k += kListThreads) {
         const uint32_t c = cnt[k];
@@ -377,7 +451,27 @@ __device__ __forceinline__ void node_list_body(const DepositArgs& a, uint32_t* _
     }
     if (!kFill) return;
     __syncthreads();
-    if (active)
+    if (per_wave) {
+        // One LDS atomic per tile reserves the wavefront's run of slots; the
+        // covering lanes take them in lane (Morton) order.
+        for (int tz = wb[2]; tz <= wb[5]; ++tz)
+            for (int ty = wb[1]; ty <= wb[4]; ++ty)
+                for (int tx = wb[0]; tx <= wb[3]; ++tx) {
+                    const bool cov = active && tx >= t0[0] && tx <= t1[0] && ty >= t0[1] && ty <= t1[1] &&
+                                     tz >= t0[2] && tz <= t1[2];
+                    const uint64_t m = __ballot(cov);
+                    if (m == 0ull) continue;
+                    const int k = ((tz - bb[2]) * by + (ty - bb[1])) * bx + (tx - bb[0]);
+                    uint32_t run = 0u;
+                    if (lane == 0) run = atomicAdd(&cnt[k], static_cast<uint32_t>(__popcll(m)));
+                    run = __shfl(run, 0);
+                    if (cov) {
+                        const uint32_t pos =
+                            base[k] + run + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+                        if (pos < a.list_cap) list[pos] = static_cast<uint32_t>(i);
+                    }
+                }
+    } else if (active) {
         for (int tz = t0[2]; tz <= t1[2]; ++tz)
             for (int ty = t0[1]; ty <= t1[1]; ++ty)
                 for (int tx = t0[0]; tx <= t1[0]; ++tx) {
@@ -385,6 +479,7 @@ __device__ __forceinline__ void node_list_body(const DepositArgs& a, uint32_t* _
                     const uint32_t pos = base[k] + atomicAdd(&cnt[k], 1u);
                     if (pos < a.list_cap) list[pos] = static_cast<uint32_t>(i);
                 }
+    }
 }
 
 template <bool kFill>
@@ -395,7 +490,8 @@ __global__ __launch_bounds__(kListThreads) void node_list_kernel(DepositArgs a, 
     __shared__ uint32_t cnt[kListTable];
     __shared__ uint32_t base[kFill ? kListTable : 1];
     __shared__ int bb[6];
-    node_list_body<kFill>(a, counters, tile_start, list, cnt, base, bb);
+    const ListPart P = list_part(a, static_cast<int64_t>(blockIdx.x) * kListThreads + threadIdx.x);
+    node_list_body<kFill>(a, P, counters, tile_start, list, cnt, base, bb);
 }
 
 // Every mesh of an nc list in one pass over the particles: the workgroup's
@@ -410,10 +506,11 @@ __global__ __launch_bounds__(kListThreads) void multi_list_kernel(MultiDeposit J
     __shared__ uint32_t cnt[kListTable];
     __shared__ uint32_t base[kFill ? kListTable : 1];
     __shared__ int bb[6];
+    const ListPart P = list_part(J.a[0], static_cast<int64_t>(blockIdx.x) * kListThreads + threadIdx.x);
     for (int j = 0; j < J.n; ++j) {
         __syncthreads();   // the previous mesh's readers of bb, cnt and base are done
         const uint32_t off = J.tile_base[j];
-        node_list_body<kFill>(J.a[j], counters + off, kFill ? tile_start + off : nullptr, list, cnt, base, bb);
+        node_list_body<kFill>(J.a[j], P, counters + off, kFill ? tile_start + off : nullptr, list, cnt, base, bb);
     }
 }
 
@@ -601,6 +698,7 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
                                                 int accumulate)
 {
     static_assert(G == 1 || G == 2 || G == 4, "group size");
+    constexpr bool kFmaFields = std::is_same<Acc, double>::value;
     constexpr int kThreads = kWaves * kWave;
     constexpr int kGroups = kWave / G;   // groups per batch
     __shared__ Acc acc[6][kPadCells];
@@ -740,6 +838,17 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
             for (int m = 0; m < G; ++m)
                 f[m] = make_float4(__shfl(fe.x, lead + m), __shfl(fe.y, lead + m), __shfl(fe.z, lead + m),
                                    __shfl(fe.w, lead + m));
+            // The fields in FP64 once per sweep (the FMA walk's operands).
+            double fd[kFmaFields ? G : 1][4];
+            if constexpr (kFmaFields) {
+#pragma unroll
+                for (int m = 0; m < G; ++m) {
+                    fd[m][0] = f[m].x;
+                    fd[m][1] = f[m].y;
+                    fd[m][2] = f[m].z;
+                    fd[m][3] = f[m].w;
+                }
+            }
             int nz = 0, cb = 0, ux = 0, uy = 0, uz = 0;
             float wxy[G];
             float wz[G][kTileMax];
@@ -802,16 +911,45 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
                         const float ker3 = wxy[m] * wz[m][iz];
                         const bool mi = !kHalo || ((gbits >> m) & 1);
                         const bool use_n = in && mi;
-                        const Acc tn = use_n ? static_cast<Acc>(ker3) : Acc(0);
-                        const Acc tc = use_n ? static_cast<Acc>(ker3 * f[m].x) : Acc(0);
-                        const Acc tx = use_n ? static_cast<Acc>(ker3 * f[m].y) : Acc(0);
-                        const Acc ty = use_n ? static_cast<Acc>(ker3 * f[m].z) : Acc(0);
-                        const Acc tz = use_n ? static_cast<Acc>(ker3 * f[m].w) : Acc(0);
                         any_n = any_n || (use_n && ker3 != 0.0f);
-                        if (m == 0) {
-                            sn = tn; sc = tc; sx = tx; sy = ty; sz = tz;
+                        if constexpr (kFmaFields) {
+                            // FP64 sums: the weight as the reference forms
+                            // it, its products with the fields exact (FMA)
+                            // rather than rounded to FP32.  A clean batch has
+                            // finite fields, so a zero weight adds zero; the
+                            // other walk selects the products instead.
+                            const double w = use_n ? static_cast<double>(ker3) : 0.0;
+                            const double fx = fd[m][0], fy = fd[m][1], fz = fd[m][2], fw = fd[m][3];
+                            if (m == 0) {
+                                sn = w;
+                                sc = kClean ? w * fx : (use_n ? w * fx : 0.0);
+                                sx = kClean ? w * fy : (use_n ? w * fy : 0.0);
+                                sy = kClean ? w * fz : (use_n ? w * fz : 0.0);
+                                sz = kClean ? w * fw : (use_n ? w * fw : 0.0);
+                            } else if (kClean) {
+                                sn += w;
+                                sc = fma(w, fx, sc);
+                                sx = fma(w, fy, sx);
+                                sy = fma(w, fz, sy);
+                                sz = fma(w, fw, sz);
+                            } else {
+                                sn += w;
+                                sc += use_n ? w * fx : 0.0;
+                                sx += use_n ? w * fy : 0.0;
+                                sy += use_n ? w * fz : 0.0;
+                                sz += use_n ? w * fw : 0.0;
+                            }
                         } else {
-                            sn += tn; sc += tc; sx += tx; sy += ty; sz += tz;
+                            const Acc tn = use_n ? static_cast<Acc>(ker3) : Acc(0);
+                            const Acc tc = use_n ? static_cast<Acc>(ker3 * f[m].x) : Acc(0);
+                            const Acc tx = use_n ? static_cast<Acc>(ker3 * f[m].y) : Acc(0);
+                            const Acc ty = use_n ? static_cast<Acc>(ker3 * f[m].z) : Acc(0);
+                            const Acc tz = use_n ? static_cast<Acc>(ker3 * f[m].w) : Acc(0);
+                            if (m == 0) {
+                                sn = tn; sc = tc; sx = tx; sy = ty; sz = tz;
+                            } else {
+                                sn += tn; sc += tc; sx += tx; sy += ty; sz += tz;
+                            }
                         }
                         if (kHalo) {
                             const bool use_h = in && !mi;
