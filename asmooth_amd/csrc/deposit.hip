@@ -8,10 +8,16 @@
 // in each tile its cell range overlaps (count, scan, fill).  One workgroup then
 // owns one tile: it accumulates every (particle, cell) contribution into FP64
 // accumulators in LDS with native ds_add_f64, and flushes the tile once with
-// plain stores - no global atomics, every cell written exactly once.  Each
-// contribution is the reference's FP32 term, bit for bit (cell range
-// coarse_mesh.cpp:72-75, weights :80-93, products :95-122); only the sums are
-// carried in FP64, so each stored cell is the FP32 rounding of an FP64 sum.
+// plain stores - no global atomics, every cell written exactly once.  The
+// cell range and each cell's weight ker3 = (wx * wy) * wz are the reference's
+// FP32 arithmetic, bit for bit (coarse_mesh.cpp:72-75, weights :80-93).  The
+// weight's products with the fields (:95-122) are the reference's FP32
+// products in the periodic variant and the FP32 study modes; the node-mesh
+// tile kernel forms them exactly, as FP64 FMAs of the FP32 weight and field,
+// instead of rounding each to FP32 (a sum moves by at most 2^-24 of the sum
+// of its terms' magnitudes, well under the reference's own FP32
+// accumulation).  The sums are carried in FP64, so each stored cell is the
+// FP32 rounding of an FP64 sum.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -676,11 +682,11 @@ __global__ __launch_bounds__(kDepThreads) void deposit_tile_kernel(
 // the sweep mark their start in a per-wavefront LDS map (tagged with the
 // sweep, so it never needs clearing), and a ballot of the marks gives every
 // lane its group.  The lane forms wx * wy of every member for its column once
-// and walks the column's cells: each member's term is the reference's FP32
-// product ((wx * wy) * wz and its products with the fields,
-// coarse_mesh.cpp:93-122), the members' terms are added in FP64 in member
-// order, and each field takes one ds_add_f64 per cell that some member
-// reaches.  Lanes of one group cover consecutive x, so one instruction's
+// and walks the column's cells: each member's weight is the reference's FP32
+// product (wx * wy) * wz (coarse_mesh.cpp:93), its field terms are FP64 FMAs
+// of that weight and the fields (exact products, :95-122), the members' terms
+// are added in FP64 in member order, and each field takes one ds_add_f64 per
+// cell that some member reaches.  Lanes of one group cover consecutive x, so one instruction's
 // atomics hit neighbouring words.
 //
 // Eight wavefronts share a tile's accumulators: the tile's padded FP64 sums
@@ -881,9 +887,9 @@ __device__ __forceinline__ void tile_group_body(const DepositArgs& a, int64_t ti
                 uz = bz;
                 cb = bz * kPadZ + uy * kPadY + ux;
             }
-            // The column's cells.  The members' terms (each the reference's
-            // FP32 product) are summed in FP64 in member order, and each
-            // field takes one atomic per cell that some member reaches.
+            // The column's cells.  The members' terms are summed in FP64 in
+            // member order, and each field takes one atomic per cell that
+            // some member reaches.
             const auto walk = [&](auto halo_tag, auto clean_tag) {
                 constexpr bool kHalo = decltype(halo_tag)::value;
                 constexpr bool kClean = decltype(clean_tag)::value;
