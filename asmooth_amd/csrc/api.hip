@@ -449,7 +449,7 @@ int asm_tree_construct(asm_context* ctx, float left, float right, int64_t* nnode
         StageTimer t(ctx, T_SORT);
         HIPC(launch_morton_keys(ctx->input.p, n, left, right, ctx->keys.p, ctx->vals.p, s));
         bool in_tmp = false;
-        HIPC(launch_radix_sort(ctx->keys.p, ctx->vals.p, ctx->keys_tmp.p, ctx->vals_tmp.p, n, 30, ctx->hist.p,
+        HIPC(launch_radix_sort(ctx->keys.p, ctx->vals.p, ctx->keys_tmp.p, ctx->vals_tmp.p, n, kMortonBits, ctx->hist.p,
                                ctx->scan_tmp.p, s, &in_tmp));
         ctx->perm = in_tmp ? ctx->vals_tmp.p : ctx->vals.p;
     }

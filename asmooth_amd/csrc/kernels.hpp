@@ -56,6 +56,12 @@ struct KnnStats {
     unsigned long long dist_evals;
     unsigned long long inserts;
 };
+// Morton keys of the tree build: kMortonAxisBits bits per axis of the tree
+// cube, interleaved into kMortonBits.  Nine bits (cells of 1/512 of the cube)
+// order the particles as well for the walks as ten and save a radix pass.
+constexpr int kMortonAxisBits = 9;
+constexpr int kMortonBits = 3 * kMortonAxisBits;
+
 // Search counters are spread over kStatSlots copies (wavefront w adds to
 // copy w % kStatSlots), so the per-wavefront atomics do not serialize on one
 // address; the host sums the copies.

@@ -162,7 +162,7 @@ extern "C" int asm_smooth_noalloc(int device, const asm_particle* d_particles, i
     // Tree (asm_tree_construct).
     NAC(launch_morton_keys(reinterpret_cast<const ParticleRec*>(d_particles), n, left, right, keys, vals, s));
     bool in_tmp = false;
-    NAC(launch_radix_sort(keys, vals, keys_tmp, vals_tmp, n, 30, hist, scan_tmp, s, &in_tmp));
+    NAC(launch_radix_sort(keys, vals, keys_tmp, vals_tmp, n, kMortonBits, hist, scan_tmp, s, &in_tmp));
     const uint32_t* perm = in_tmp ? vals_tmp : vals;
     NAC(launch_gather_sorted(reinterpret_cast<const ParticleRec*>(d_particles), perm, n, spos, sattr, inv_perm, s));
     NAC(launch_build_tree(spos, n, kLeaf, nleaf, nleaf_pad, nodes, groups, s));

@@ -1,7 +1,7 @@
 // GPU tree build.
 //
 // Replaces KDTreeSimple::construct (kdtree_balanced.cpp:212-296).  Particles
-// are put in Morton order (10 bits per axis over the root cube the reference
+// are put in Morton order (9 bits per axis over the root cube the reference
 // passes, clumping_tree_main.cpp:392-394) by a stable radix sort, then cut into
 // leaves of `leaf_size` consecutive particles.  Over the leaves sits an
 // implicit complete binary tree (node i has children 2i and 2i+1, leaves at
@@ -15,7 +15,8 @@
 namespace asm_hip {
 namespace {
 
-// Spread the low 10 bits of v so that bit i moves to bit 3i.
+// Spread the low 10 bits of v so that bit i moves to bit 3i (keys use the
+// low kMortonAxisBits).
 __device__ __forceinline__ uint32_t spread3(uint32_t v)
 {
     v &= 0x3ffu;
@@ -36,7 +37,7 @@ __global__ void morton_keys_kernel(const ParticleRec* __restrict__ p, int64_t n,
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float f = (p[i].x[a] - left) * scale;
-        f = fminf(fmaxf(f, 0.0f), 1023.0f);
+        f = fminf(fmaxf(f, 0.0f), static_cast<float>((1 << kMortonAxisBits) - 1));
         c[a] = static_cast<uint32_t>(f);
     }
     keys[i] = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
@@ -150,7 +151,7 @@ hipError_t launch_morton_keys(const ParticleRec* p, int64_t n, float left, float
                               uint32_t* keys, uint32_t* vals, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const float scale = 1024.0f / (right - left);
+    const float scale = static_cast<float>(1 << kMortonAxisBits) / (right - left);
     const int th = 256;
     morton_keys_kernel<<<static_cast<unsigned>((n + th - 1) / th), th, 0, s>>>(p, n, left, scale, keys, vals);
     return hipGetLastError();

@@ -440,9 +440,14 @@ def test_global_mesh_variant_matches_oracle(ctx, coord):
         assert abs(mesh_fields(gm, nc)["n"].sum() - nl) < 1e-3 * nl
 
 
+MORTON_AXIS_BITS = 9   # kMortonAxisBits of the tree build
+
+
 def _morton(pos, left, right):
-    scale = np.float32(1024.0) / (np.float32(right) - np.float32(left))
-    c = np.clip((pos.astype(np.float32) - np.float32(left)) * scale, np.float32(0), np.float32(1023)).astype(np.uint32)
+    cells = 1 << MORTON_AXIS_BITS
+    scale = np.float32(cells) / (np.float32(right) - np.float32(left))
+    c = np.clip((pos.astype(np.float32) - np.float32(left)) * scale, np.float32(0),
+                np.float32(cells - 1)).astype(np.uint32)
 
     def spread(v):
         v = v & 0x3FF
