@@ -297,7 +297,7 @@ constexpr int kListTable = 1024;
 // them by ballots (one LDS atomic per tile), instead of one LDS atomic per
 // (particle, tile) pair: Morton-consecutive particles share tiles, and
 // same-word LDS atomics serialise.
-constexpr int kListWaveTiles = 32;
+constexpr int kListWaveTiles = 16;
 
 // A node-mesh particle's (x -+ h) / boxsize per axis: the part of its cell
 // range (axis_range, coarse_mesh.cpp:72-75) that does not depend on nc, so
